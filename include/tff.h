@@ -1,0 +1,154 @@
+/*
+ * tff.h -- C ABI of the MI355X force-free step.
+ *
+ * The reference (tristan-ff) exposes its hot path as Fortran module
+ * procedures acting on module-global arrays; it has no C interface.  Each
+ * entry point below replaces one of those procedures and keeps its meaning,
+ * so that a Fortran host binds them through ISO_C_BINDING (see
+ * tristan-ff_amd/fortran/m_tff.F90) and keeps the reference's call order.
+ *
+ * Conventions
+ *  - Every function returns TFF_OK (0) or a negative TFF_E* code; nothing
+ *    aborts.  tff_last_error() gives the text of the last failure.
+ *  - A handle owns all device memory of one meshblock and one HIP stream.
+ *    Calls are asynchronous on that stream except tff_upload, tff_download,
+ *    tff_reduce and tff_sync, which return after the stream has drained.
+ *  - A handle is not thread-safe: call it from one host thread.
+ *  - Host arrays passed to tff_upload / tff_download hold one field
+ *    component in the reference's layout: Fortran order (x fastest),
+ *    indices -2..n-1+2 on every axis, i.e. (nx+4)*(ny+4)*(nz+4) values
+ *    starting at element (-2,-2,-2) (src/objects/fields.F90:14-27,
+ *    src/logistics/initialize.F90:192-199; NGHOST=2, src/CMakeLists.txt:19).
+ */
+#ifndef TFF_H
+#define TFF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TFF_NGHOST 2
+
+/* return codes */
+#define TFF_OK 0
+#define TFF_EINVAL -1   /* bad argument or configuration */
+#define TFF_EHIP -2     /* a HIP runtime call failed */
+#define TFF_ENOMEM -3   /* device allocation failed */
+#define TFF_ESTATE -4   /* call not valid in the handle's current state */
+
+/* precision of the path (the reference computes in default REAL = FP32,
+ * CMakeLists.txt:23,25) */
+#define TFF_FP64 0      /* FP64 storage and arithmetic */
+#define TFF_FP32 1      /* FP32 everywhere, operation order of the reference */
+#define TFF_FP32_BULK 2 /* FP32 storage and stencils; FP64 for |B|^2
+                           normalisation, the E.B projection, the stage
+                           accumulation and all reductions */
+
+/* boundary flags, src/objects/domain.F90:31-33 and initialize.F90:139-148 */
+#define TFF_OPEN 0
+#define TFF_PERIODIC 1
+#define TFF_ABSORB 2
+
+/* field component ids for upload/download (src/objects/fields.F90:14-27) */
+#define TFF_EX 0
+#define TFF_EY 1
+#define TFF_EZ 2
+#define TFF_BX 3
+#define TFF_BY 4
+#define TFF_BZ 5
+#define TFF_B0X 6
+#define TFF_B0Y 7
+#define TFF_B0Z 8
+#define TFF_ENX 9
+#define TFF_ENY 10
+#define TFF_ENZ 11
+#define TFF_BNX 12
+#define TFF_BNY 13
+#define TFF_BNZ 14
+
+/* host element types */
+#define TFF_HOST_F64 0
+#define TFF_HOST_F32 1
+
+/* boundary-condition kernels (userFieldBoundaryConditions of the user files) */
+#define TFF_BC_NONE 0    /* user_emwave.F90:75-78, user_dummy.F90:312-315 */
+#define TFF_BC_LOOP 1    /* user_loop.F90:181-244 */
+#define TFF_BC_PSR 2     /* user_psr.F90:72-332 */
+#define TFF_BC_UNIFORM 3 /* user_uniform.F90:77-147 */
+
+typedef struct tff_handle tff_handle;
+
+typedef struct tff_config {
+  int nx, ny, nz;      /* interior cells: this_meshblock%ptr%{sx,sy,sz} */
+  int boundary[3];     /* boundary_x/y/z: TFF_OPEN, TFF_PERIODIC, TFF_ABSORB;
+                          any TFF_ABSORB makes all three absorbing as
+                          initialize.F90:143-148 does */
+  int absorb_buff;     /* <grid> absorb_buff (default 20, initialize.F90:147) */
+  int absorb_fixed;    /* 0: rk3Update's absorbing layer as coded
+                          (field_solver.F90:211 uses i for jg; delta
+                          unclamped); 1: jg from j and delta clamped to 1 */
+  int precision;       /* TFF_FP64, TFF_FP32, TFF_FP32_BULK */
+  int device;          /* HIP device ordinal */
+  double cc;           /* <algorithm> c, the CFL number CC (global_namespace.F90:14) */
+} tff_config;
+
+/* Allocate a meshblock on the device.  Replaces the field part of
+ * initializeFields (initialize.F90:201-240); all arrays start at zero. */
+int tff_create(const tff_config* cfg, tff_handle** out);
+void tff_destroy(tff_handle* h);
+const char* tff_last_error(const tff_handle* h);
+
+/* Copy one component between a host array (layout above) and the device.
+ * Replaces direct access to the m_fields module arrays by userInitFields
+ * (e.g. user_psr.F90:39-69) and by the output writer. */
+int tff_upload(tff_handle* h, int field, const void* host, int host_dtype);
+int tff_download(tff_handle* h, int field, void* host, int host_dtype);
+
+/* Register the problem's boundary kernel and its parameters (the values the
+ * user file's userReadInput computes; the order is listed per kind in
+ * INTEGRATION.md). */
+int tff_set_bc(tff_handle* h, int kind, const double* params, int nparams);
+
+/* field_solver.F90:15-33 initRKstep: En,Bn <- E,B. */
+int tff_init_rk_step(tff_handle* h);
+/* field_solver.F90:35-42 rk3Step(rk_c1, rk_c2, rk_c3): computeRho, rk3Push
+ * and rk3Update in one pass.  U <- c1*Un + c2*U + c3*L(U) on the interior. */
+int tff_rk3_step(tff_handle* h, double c1, double c2, double c3);
+/* field_solver.F90:273-335 cleanEpar: remove E parallel to B (interior). */
+int tff_clean_epar(tff_handle* h);
+/* exchange_fields.F90:11-197 exchangeFields([exchangeB0]) on one meshblock:
+ * ghost layers of periodic axes take the wrapped interior; ghosts of
+ * non-periodic axes are left as they are. */
+int tff_exchange_fields(tff_handle* h, int exchange_b0);
+/* userFieldBoundaryConditions(step) of the registered problem. */
+int tff_user_bc(tff_handle* h, int step);
+
+/* One whole timestep in the order of tristanmainloop.F90:50-83 (three
+ * stages, each followed by BC, ghost fill, cleanEpar, ghost fill). */
+int tff_step(tff_handle* h, int timestep);
+/* nsteps timesteps first_step, first_step+1, ... replayed from a captured
+ * HIP graph. */
+int tff_run(tff_handle* h, int first_step, int nsteps);
+
+/* FP64 diagnostics over the interior (no reference counterpart):
+ * out[0] = 1/2 sum E^2, out[1] = 1/2 sum B^2, out[2] = max |div_h B|,
+ * out[3] = sqrt(mean (div_h B)^2). */
+int tff_reduce(tff_handle* h, double out[4]);
+
+/* Wait for all work queued on the handle's stream. */
+int tff_sync(tff_handle* h);
+
+/* Timing on the handle's stream (for benchmarks): record event slot
+ * 0..15, then read the time between two recorded slots in milliseconds. */
+int tff_event_record(tff_handle* h, int slot);
+int tff_event_elapsed(tff_handle* h, int slot_a, int slot_b, float* ms);
+/* When enabled, every launch of the stage kernel is bracketed by HIP events
+ * on the handle's stream and tff_stage_kernel_time() returns the summed
+ * kernel time and the number of launches since the last reset. */
+int tff_set_kernel_timing(tff_handle* h, int on);
+int tff_stage_kernel_time(tff_handle* h, double* total_ms, long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TFF_H */

@@ -1,0 +1,72 @@
+"""Build libtff.so (HIP kernels + C ABI) for gfx950, in-tree.
+
+    python tristan-ff_amd/build.py            # libtff.so
+    python tristan-ff_amd/build.py --fortran  # also the Fortran host (tristan_ff)
+
+The library is written to tristan-ff_amd/_build/libtff.so, next to the
+sources, so that it travels with the tree to the machine that runs it.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+OUT = os.path.join(PKG, "_build")
+LIB = os.path.join(OUT, "libtff.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# -ffp-contract=off: no fused multiply-add, so the FP64/FP32 paths round
+# exactly where the reference's expressions round.  Packed FP32 arithmetic
+# is disabled for every device function.
+HIP_FLAGS = [
+    "-O3", f"--offload-arch={ARCH}", "-fPIC", "-shared", "-std=c++17",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(ROOT, "include"),
+    "-I", os.path.join(PKG, "csrc"),
+]
+
+
+def build_lib(verbose: bool = False) -> str:
+    os.makedirs(OUT, exist_ok=True)
+    src = os.path.join(PKG, "csrc", "tff.hip")
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, *HIP_FLAGS, src, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Fortran host (amdflang): deck parser, main loop and user files that
+    call libtff.so through ISO_C_BINDING."""
+    fdir = os.path.join(PKG, "fortran")
+    mk = os.path.join(fdir, "Makefile")
+    if not os.path.exists(mk):
+        return ""
+    subprocess.run(["make", "-s", "-C", fdir], check=True)
+    return os.path.join(OUT, "tristan_ff")
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--fortran", action="store_true")
+    ap.add_argument("-v", "--verbose", action="store_true")
+    a = ap.parse_args(argv)
+    print(build_lib(a.verbose))
+    if a.fortran and shutil.which("amdflang"):
+        print(build_fortran(a.verbose))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,678 @@
+// tff.hip -- host side of libtff.so: the C ABI declared in include/tff.h.
+//
+// A handle holds one meshblock in HBM as structure-of-arrays buffers (one
+// per field component, layout in tff_common.h) and one HIP stream.  The
+// reference's module arrays ex..bz are the handle's state buffers; each
+// kernel writes a fresh buffer and the host swaps pointers, so no kernel
+// ever reads what another thread of the same launch writes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "tff.h"
+#include "tff_kernels.h"
+
+using namespace tff;
+
+namespace {
+
+struct Vec {
+  void* p[3] = {nullptr, nullptr, nullptr};
+};
+
+template <class S>
+V3<S> mv3(const Vec& v) {
+  return V3<S>{(S*)v.p[0], (S*)v.p[1], (S*)v.p[2]};
+}
+template <class S>
+V3<const S> cv3(const Vec& v) {
+  return V3<const S>{(const S*)v.p[0], (const S*)v.p[1], (const S*)v.p[2]};
+}
+
+}  // namespace
+
+struct tff_handle {
+  tff_config cfg{};
+  size_t esz = 8;  // storage element size
+  Geom g{};
+  int xo = 0;           // elements before the first interior cell of a row
+  int64_t nalloc = 0;   // elements per component buffer
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // state (the reference's ex..bz), stage work buffers and the clean target
+  Vec es, bs, w1e, w1b, w2e, w2b, ce;
+  Vec en, bn;  // initRKstep copies (allocated on first use)
+  Vec b0, t0;  // background field and its curl bracket
+  bool t0_dirty = true;
+  int has_t0 = 0;
+  int* d_flag = nullptr;
+
+  int bc_kind = TFF_BC_NONE;
+  double bc_host[16] = {0};
+  double* d_bc = nullptr;
+  void* psr_stage = nullptr;
+  int3 psr_lo{0, 0, 0}, psr_n{0, 0, 0};
+
+  int* d_step = nullptr;
+  double* d_part = nullptr;
+  double* d_out = nullptr;
+  int red_blocks = 0;
+
+  hipEvent_t ev[16] = {};
+  bool timing = false;
+  std::vector<hipEvent_t> tpool;
+  size_t tused = 0;
+  double tsum_ms = 0.0;
+  long tcount = 0;
+
+  std::map<void*, hipGraphExec_t> graphs;  // keyed by the state E buffer
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+int fail(tff_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail((h), TFF_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+int alloc_vec(tff_handle* h, Vec& v) {
+  for (int c = 0; c < 3; ++c) {
+    void* p = nullptr;
+    if (hipMalloc(&p, (size_t)h->nalloc * h->esz) != hipSuccess)
+      return fail(h, TFF_ENOMEM, "hipMalloc of a field buffer failed");
+    if (hipMemset(p, 0, (size_t)h->nalloc * h->esz) != hipSuccess)
+      return fail(h, TFF_EHIP, "hipMemset failed");
+    h->allocs.push_back(p);
+    v.p[c] = p;
+  }
+  return TFF_OK;
+}
+
+void invalidate_graphs(tff_handle* h) {
+  for (auto& kv : h->graphs) hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+}
+
+int last_launch(tff_handle* h, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(h, TFF_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  return TFF_OK;
+}
+
+dim3 box_grid(const tff_handle* h) {
+  const int sx = h->g.nx + 2 * NG, sy = h->g.ny + 2 * NG, sz = h->g.nz + 2 * NG;
+  return dim3((sx + BX - 1) / BX, (sy + BY - 1) / BY, sz);
+}
+
+// ---- precision-dispatched launches ---------------------------------------
+template <class PR>
+int launch_rk3(tff_handle* h, const Vec& xe, const Vec& xb, const Vec& ue, const Vec& ub,
+               const Vec& ye, const Vec& yb, double c1, double c2, double c3) {
+  using S = typename PR::S;
+  using P = typename PR::P;
+  Absorb<P> ab{};
+  ab.on = h->cfg.boundary[0] == TFF_ABSORB;
+  ab.fixed = h->cfg.absorb_fixed;
+  // field_solver.F90:200-204, evaluated in the working precision
+  ab.xc = (P)0.5 * (P)h->g.nx;
+  ab.yc = (P)0.5 * (P)h->g.ny;
+  ab.zc = (P)0.5 * (P)h->g.nz;
+  P rmax = std::min(std::min(ab.xc, ab.yc), ab.zc);
+  ab.rabsorb = rmax - (P)h->cfg.absorb_buff;
+  ab.buff = (P)h->cfg.absorb_buff;
+  // FP64 and FP32 take CC and the stage coefficients in their own precision
+  // (the reference's default-REAL literals); FP32-bulk accumulates in FP64
+  // with the exact FP64 values.
+  const bool exact = std::is_same<S, P>::value;
+  const P cc = exact ? (P)(S)h->cfg.cc : (P)h->cfg.cc;
+  const P k1 = exact ? (P)(S)c1 : (P)c1, k2 = exact ? (P)(S)c2 : (P)c2, k3 = exact ? (P)(S)c3 : (P)c3;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (h->timing) {
+    while (h->tpool.size() < h->tused + 2) {
+      hipEvent_t e;
+      HIPCHK(h, hipEventCreate(&e));
+      h->tpool.push_back(e);
+    }
+    e0 = h->tpool[h->tused++];
+    e1 = h->tpool[h->tused++];
+    HIPCHK(h, hipEventRecord(e0, h->stream));
+  }
+  hipLaunchKernelGGL(k_rk3<PR>, box_grid(h), dim3(BX, BY), 0, h->stream, h->g, cv3<S>(xe), cv3<S>(xb),
+                     cv3<S>(ue), cv3<S>(ub), cv3<S>(h->t0), h->has_t0, mv3<S>(ye), mv3<S>(yb), cc,
+                     k1, k2, k3, ab);
+  int rc = last_launch(h, "k_rk3");
+  if (rc) return rc;
+  if (h->timing) HIPCHK(h, hipEventRecord(e1, h->stream));
+  return TFF_OK;
+}
+
+template <class PR>
+int launch_clean(tff_handle* h, const Vec& xe, const Vec& xb, const Vec& ye) {
+  using S = typename PR::S;
+  hipLaunchKernelGGL(k_clean<PR>, box_grid(h), dim3(BX, BY), 0, h->stream, h->g, cv3<S>(xe),
+                     cv3<S>(xb), mv3<S>(ye));
+  return last_launch(h, "k_clean");
+}
+
+template <class S>
+int launch_fill(tff_handle* h, std::initializer_list<const Vec*> vecs) {
+  const int* b = h->cfg.boundary;
+  if (b[0] != TFF_PERIODIC && b[1] != TFF_PERIODIC && b[2] != TFF_PERIODIC) return TFF_OK;
+  FillSet<S> fs{};
+  fs.n = 0;
+  for (const Vec* v : vecs)
+    for (int c = 0; c < 3; ++c) fs.a[fs.n++] = (S*)v->p[c];
+  const int sx = h->g.nx + 2 * NG, sy = h->g.ny + 2 * NG;
+  const int64_t nshell = (int64_t)2 * NG * sx * sy + (int64_t)2 * NG * sx * h->g.nz +
+                         (int64_t)2 * NG * h->g.ny * h->g.nz;
+  const int tb = 256;
+  hipLaunchKernelGGL(k_fill<S>, dim3((unsigned)((nshell + tb - 1) / tb)), dim3(tb), 0, h->stream,
+                     h->g, b[0] == TFF_PERIODIC, b[1] == TFF_PERIODIC, b[2] == TFF_PERIODIC, fs,
+                     nshell);
+  return last_launch(h, "k_fill");
+}
+
+template <class PR>
+int launch_bc(tff_handle* h, const Vec& e, const Vec& b, const int* step_ptr, int step_val) {
+  using S = typename PR::S;
+  using P = typename PR::P;
+  const int sx = h->g.nx + 2 * NG, sy = h->g.ny + 2 * NG;
+  const int tb = 256;
+  const P cc = std::is_same<S, P>::value ? (P)(S)h->cfg.cc : (P)h->cfg.cc;
+  switch (h->bc_kind) {
+    case TFF_BC_LOOP: {
+      const int64_t n = (int64_t)2 * NG * sx * sy;
+      hipLaunchKernelGGL(k_bc_loop<PR>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, h->stream,
+                         h->g, mv3<S>(e), mv3<S>(b), h->d_bc, cc, step_ptr, step_val);
+      return last_launch(h, "k_bc_loop");
+    }
+    case TFF_BC_UNIFORM: {
+      const int64_t n = (int64_t)(2 * NG + 1) * sx * sy;
+      hipLaunchKernelGGL(k_bc_uniform<PR>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0,
+                         h->stream, h->g, mv3<S>(e), mv3<S>(b), h->d_bc, cc, step_ptr, step_val);
+      return last_launch(h, "k_bc_uniform");
+    }
+    case TFF_BC_PSR: {
+      const int64_t n = (int64_t)h->psr_n.x * h->psr_n.y * h->psr_n.z;
+      if (n <= 0) return TFF_OK;
+      const P ccinv = (P)1.0 / cc;
+      hipLaunchKernelGGL(k_bc_psr_eval<PR>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0,
+                         h->stream, h->g, cv3<S>(e), cv3<S>(b), h->d_bc, ccinv, h->psr_lo, h->psr_n,
+                         (S*)h->psr_stage, step_ptr, step_val);
+      int rc = last_launch(h, "k_bc_psr_eval");
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_bc_psr_commit<PR>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0,
+                         h->stream, h->g, mv3<S>(e), mv3<S>(b), h->d_bc, h->psr_lo, h->psr_n,
+                         (const S*)h->psr_stage);
+      return last_launch(h, "k_bc_psr_commit");
+    }
+    default:
+      return TFF_OK;
+  }
+}
+
+template <class PR>
+int ensure_t0(tff_handle* h) {
+  using S = typename PR::S;
+  if (!h->t0_dirty) return TFF_OK;
+  if (!h->t0.p[0]) {
+    int rc = alloc_vec(h, h->t0);
+    if (rc) return rc;
+  }
+  HIPCHK(h, hipMemsetAsync(h->d_flag, 0, sizeof(int), h->stream));
+  dim3 grid((h->g.nx + BX - 1) / BX, (h->g.ny + BY - 1) / BY, h->g.nz);
+  hipLaunchKernelGGL(k_b0_curl<S>, grid, dim3(BX, BY), 0, h->stream, h->g, cv3<S>(h->b0),
+                     mv3<S>(h->t0), h->d_flag);
+  int rc = last_launch(h, "k_b0_curl");
+  if (rc) return rc;
+  int flag = 0;
+  HIPCHK(h, hipMemcpyAsync(&flag, h->d_flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (flag != h->has_t0) invalidate_graphs(h);
+  h->has_t0 = flag;
+  h->t0_dirty = false;
+  return TFF_OK;
+}
+
+// One timestep in the order of tristanmainloop.F90:50-83.  X denotes the
+// cleaned fields a stage reads, Y the raw stage output.
+template <class PR>
+int enqueue_step(tff_handle* h, const int* step_ptr, int step_val) {
+  using S = typename PR::S;
+  const double* cf = nullptr;
+  static const double c64[9] = {1.0, 0.0, 1.0, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0,
+                                1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0};
+  cf = c64;  // rounded to S inside launch_rk3: FP32 gets 1.0/3.0 as float
+  int rc;
+  // stage 1: Y1 = Un + L(Un)
+  if ((rc = launch_rk3<PR>(h, h->es, h->bs, h->es, h->bs, h->w1e, h->w1b, cf[0], cf[1], cf[2]))) return rc;
+  if ((rc = launch_bc<PR>(h, h->w1e, h->w1b, step_ptr, step_val))) return rc;
+  if ((rc = launch_fill<S>(h, {&h->w1e, &h->w1b}))) return rc;
+  if ((rc = launch_clean<PR>(h, h->w1e, h->w1b, h->ce))) return rc;
+  if ((rc = launch_fill<S>(h, {&h->ce}))) return rc;
+  // stage 2: Y2 = 3/4 Un + 1/4 X1 + 1/4 L(X1)
+  if ((rc = launch_rk3<PR>(h, h->ce, h->w1b, h->es, h->bs, h->w2e, h->w2b, cf[3], cf[4], cf[5]))) return rc;
+  if ((rc = launch_bc<PR>(h, h->w2e, h->w2b, step_ptr, step_val))) return rc;
+  if ((rc = launch_fill<S>(h, {&h->w2e, &h->w2b}))) return rc;
+  if ((rc = launch_clean<PR>(h, h->w2e, h->w2b, h->ce))) return rc;
+  if ((rc = launch_fill<S>(h, {&h->ce}))) return rc;
+  // stage 3: Y3 = 1/3 Un + 2/3 X2 + 2/3 L(X2), written over Un in place
+  // (each thread reads Un only at its own cell before writing it)
+  if ((rc = launch_rk3<PR>(h, h->ce, h->w2b, h->es, h->bs, h->es, h->bs, cf[6], cf[7], cf[8]))) return rc;
+  if ((rc = launch_bc<PR>(h, h->es, h->bs, step_ptr, step_val))) return rc;
+  if ((rc = launch_fill<S>(h, {&h->es, &h->bs}))) return rc;
+  if ((rc = launch_clean<PR>(h, h->es, h->bs, h->ce))) return rc;
+  if ((rc = launch_fill<S>(h, {&h->ce}))) return rc;
+  return TFF_OK;
+}
+
+// ---- precision dispatch ----------------------------------------------------
+template <template <class> class F, class... A>
+int dispatch(tff_handle* h, A... a) {
+  switch (h->cfg.precision) {
+    case TFF_FP64: return F<PrecF64>::run(h, a...);
+    case TFF_FP32: return F<PrecF32>::run(h, a...);
+    default: return F<PrecF32Bulk>::run(h, a...);
+  }
+}
+
+template <class PR>
+struct DoStep {
+  static int run(tff_handle* h, int step, int use_graph) {
+    int rc = ensure_t0<PR>(h);
+    if (rc) return rc;
+    if (!use_graph || h->timing) {
+      rc = enqueue_step<PR>(h, nullptr, step);
+      if (rc) return rc;
+    } else {
+      void* key = h->es.p[0];
+      auto it = h->graphs.find(key);
+      if (it == h->graphs.end()) {
+        hipGraph_t graph;
+        HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_step<PR>(h, h->d_step, 0);
+        if (rc == TFF_OK) hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(64), 0, h->stream, h->d_step);
+        hipError_t e = hipStreamEndCapture(h->stream, &graph);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(h, TFF_EHIP, "hipStreamEndCapture failed");
+        hipGraphExec_t exec;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(h, TFF_EHIP, "hipGraphInstantiate failed");
+        it = h->graphs.emplace(key, exec).first;
+      }
+      HIPCHK(h, hipGraphLaunch(it->second, h->stream));
+    }
+    std::swap(h->es, h->ce);
+    return TFF_OK;
+  }
+};
+
+template <class PR>
+struct DoRk3 {
+  static int run(tff_handle* h, double c1, double c2, double c3) {
+    int rc = ensure_t0<PR>(h);
+    if (rc) return rc;
+    rc = launch_rk3<PR>(h, h->es, h->bs, h->en, h->bn, h->w1e, h->w1b, c1, c2, c3);
+    if (rc) return rc;
+    std::swap(h->es, h->w1e);
+    std::swap(h->bs, h->w1b);
+    invalidate_graphs(h);
+    return TFF_OK;
+  }
+};
+
+template <class PR>
+struct DoClean {
+  static int run(tff_handle* h) {
+    int rc = launch_clean<PR>(h, h->es, h->bs, h->ce);
+    if (rc) return rc;
+    std::swap(h->es, h->ce);
+    return TFF_OK;
+  }
+};
+
+template <class PR>
+struct DoBC {
+  static int run(tff_handle* h, int step) { return launch_bc<PR>(h, h->es, h->bs, nullptr, step); }
+};
+
+template <class PR>
+struct DoFill {
+  static int run(tff_handle* h, int b0) {
+    using S = typename PR::S;
+    if (b0) {
+      h->t0_dirty = true;
+      return launch_fill<S>(h, {&h->es, &h->bs, &h->b0});
+    }
+    return launch_fill<S>(h, {&h->es, &h->bs});
+  }
+};
+
+template <class PR>
+struct DoReduce {
+  static int run(tff_handle* h, double* out) {
+    using S = typename PR::S;
+    hipLaunchKernelGGL(k_reduce_partial<S>, dim3(h->red_blocks), dim3(RED_THREADS), 0, h->stream,
+                       h->g, cv3<S>(h->es), cv3<S>(h->bs), h->d_part);
+    int rc = last_launch(h, "k_reduce_partial");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, h->stream, h->d_part, h->red_blocks,
+                       h->d_out, (double)h->g.nx * h->g.ny * h->g.nz);
+    if ((rc = last_launch(h, "k_reduce_final"))) return rc;
+    HIPCHK(h, hipMemcpyAsync(out, h->d_out, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return TFF_OK;
+  }
+};
+
+Vec* field_vec(tff_handle* h, int field, int& comp) {
+  comp = field % 3;
+  switch (field / 3) {
+    case 0: return &h->es;
+    case 1: return &h->bs;
+    case 2: return &h->b0;
+    case 3: return &h->en;
+    case 4: return &h->bn;
+    default: return nullptr;
+  }
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+int tff_create(const tff_config* cfg, tff_handle** out) {
+  if (!cfg || !out) return TFF_EINVAL;
+  *out = nullptr;
+  if (cfg->nx < NG || cfg->ny < NG || cfg->nz < NG) return TFF_EINVAL;  // initialize.F90:256-264
+  if (cfg->precision < TFF_FP64 || cfg->precision > TFF_FP32_BULK) return TFF_EINVAL;
+  for (int a = 0; a < 3; ++a)
+    if (cfg->boundary[a] < 0 || cfg->boundary[a] > 2) return TFF_EINVAL;
+  if (!(cfg->cc > 0.0) || cfg->absorb_buff < 0) return TFF_EINVAL;
+  tff_handle* h = new tff_handle();
+  h->cfg = *cfg;
+  // initialize.F90:143-148: one absorbing axis makes all three absorbing
+  if (cfg->boundary[0] == TFF_ABSORB || cfg->boundary[1] == TFF_ABSORB || cfg->boundary[2] == TFF_ABSORB)
+    h->cfg.boundary[0] = h->cfg.boundary[1] = h->cfg.boundary[2] = TFF_ABSORB;
+  h->esz = cfg->precision == TFF_FP64 ? 8 : 4;
+  const int al = (int)(128 / h->esz);  // elements per 128-byte line
+  h->xo = al;
+  h->g.nx = cfg->nx;
+  h->g.ny = cfg->ny;
+  h->g.nz = cfg->nz;
+  h->g.px = ((h->xo + cfg->nx + NG + al - 1) / al) * al;
+  h->g.plane = (int64_t)h->g.px * (cfg->ny + 2 * NG);
+  h->g.org = h->xo + (int64_t)NG * h->g.px + (int64_t)NG * h->g.plane;
+  h->nalloc = h->g.plane * (cfg->nz + 2 * NG) + al;
+  if (hipSetDevice(cfg->device) != hipSuccess) {
+    delete h;
+    return TFF_EHIP;
+  }
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return TFF_EHIP;
+  }
+  int rc = TFF_OK;
+  Vec* vs[] = {&h->es, &h->bs, &h->w1e, &h->w1b, &h->w2e, &h->w2b, &h->ce, &h->b0};
+  for (Vec* v : vs)
+    if ((rc = alloc_vec(h, *v))) break;
+  h->red_blocks = 1024;
+  if (!rc && (hipMalloc(&h->d_step, sizeof(int)) != hipSuccess ||
+              hipMalloc(&h->d_flag, sizeof(int)) != hipSuccess ||
+              hipMalloc(&h->d_bc, 16 * sizeof(double)) != hipSuccess ||
+              hipMalloc(&h->d_part, 4 * sizeof(double) * h->red_blocks) != hipSuccess ||
+              hipMalloc(&h->d_out, 4 * sizeof(double)) != hipSuccess))
+    rc = TFF_ENOMEM;
+  if (!rc && hipMemset(h->d_bc, 0, 16 * sizeof(double)) != hipSuccess) rc = TFF_EHIP;
+  for (int s = 0; !rc && s < 16; ++s)
+    if (hipEventCreate(&h->ev[s]) != hipSuccess) rc = TFF_EHIP;
+  if (rc) {
+    tff_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return TFF_OK;
+}
+
+void tff_destroy(tff_handle* h) {
+  if (!h) return;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  invalidate_graphs(h);
+  for (void* p : h->allocs) hipFree(p);
+  if (h->psr_stage) hipFree(h->psr_stage);
+  hipFree(h->d_step);
+  hipFree(h->d_flag);
+  hipFree(h->d_bc);
+  hipFree(h->d_part);
+  hipFree(h->d_out);
+  for (auto e : h->ev)
+    if (e) hipEventDestroy(e);
+  for (auto e : h->tpool) hipEventDestroy(e);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+const char* tff_last_error(const tff_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+static int xfer(tff_handle* h, int field, void* host, int host_dtype, bool up) {
+  if (!h || !host) return TFF_EINVAL;
+  if (field < TFF_EX || field > TFF_BNZ) return fail(h, TFF_EINVAL, "unknown field id");
+  if (host_dtype != TFF_HOST_F64 && host_dtype != TFF_HOST_F32)
+    return fail(h, TFF_EINVAL, "unknown host dtype");
+  int comp;
+  Vec* v = field_vec(h, field, comp);
+  if (field >= TFF_ENX && !v->p[0]) {
+    if (!up) return fail(h, TFF_ESTATE, "En/Bn are not allocated before tff_init_rk_step");
+    int rc = alloc_vec(h, *v);
+    if (rc) return rc;
+  }
+  const int sx = h->g.nx + 2 * NG, sy = h->g.ny + 2 * NG, sz = h->g.nz + 2 * NG;
+  const size_t n = (size_t)sx * sy * sz;
+  const size_t hsz = host_dtype == TFF_HOST_F64 ? 8 : 4;
+  std::vector<unsigned char> tmp;
+  void* hp = host;
+  if (hsz != h->esz) {
+    tmp.resize(n * h->esz);
+    hp = tmp.data();
+    if (up) {
+      if (h->esz == 4)
+        for (size_t t = 0; t < n; ++t) ((float*)hp)[t] = (float)((const double*)host)[t];
+      else
+        for (size_t t = 0; t < n; ++t) ((double*)hp)[t] = (double)((const float*)host)[t];
+    }
+  }
+  char* dbase = (char*)v->p[comp] + (size_t)(h->xo - NG) * h->esz;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (up)
+    HIPCHK(h, hipMemcpy2D(dbase, (size_t)h->g.px * h->esz, hp, (size_t)sx * h->esz, (size_t)sx * h->esz,
+                          (size_t)sy * sz, hipMemcpyHostToDevice));
+  else
+    HIPCHK(h, hipMemcpy2D(hp, (size_t)sx * h->esz, dbase, (size_t)h->g.px * h->esz, (size_t)sx * h->esz,
+                          (size_t)sy * sz, hipMemcpyDeviceToHost));
+  if (!up && hsz != h->esz) {
+    if (h->esz == 4)
+      for (size_t t = 0; t < n; ++t) ((double*)host)[t] = (double)((const float*)hp)[t];
+    else
+      for (size_t t = 0; t < n; ++t) ((float*)host)[t] = (float)((const double*)hp)[t];
+  }
+  if (up && field >= TFF_B0X && field <= TFF_B0Z) h->t0_dirty = true;
+  return TFF_OK;
+}
+
+int tff_upload(tff_handle* h, int field, const void* host, int host_dtype) {
+  return xfer(h, field, const_cast<void*>(host), host_dtype, true);
+}
+
+int tff_download(tff_handle* h, int field, void* host, int host_dtype) {
+  return xfer(h, field, host, host_dtype, false);
+}
+
+int tff_set_bc(tff_handle* h, int kind, const double* params, int nparams) {
+  if (!h) return TFF_EINVAL;
+  if (kind < TFF_BC_NONE || kind > TFF_BC_UNIFORM) return fail(h, TFF_EINVAL, "unknown BC kind");
+  static const int need[4] = {0, 9, 6, 6};
+  if (nparams < need[kind] || nparams > 16 || (nparams > 0 && !params))
+    return fail(h, TFF_EINVAL, "wrong number of BC parameters");
+  std::memset(h->bc_host, 0, sizeof(h->bc_host));
+  for (int t = 0; t < nparams; ++t) h->bc_host[t] = params[t];
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(h->d_bc, h->bc_host, 16 * sizeof(double), hipMemcpyHostToDevice));
+  if (h->psr_stage) {
+    hipFree(h->psr_stage);
+    h->psr_stage = nullptr;
+  }
+  h->psr_n = int3{0, 0, 0};
+  if (kind == TFF_BC_PSR) {
+    // bounding box of the supersphere r < 2R (user_psr.F90:91,153), clipped
+    // to the interior the BC loops over
+    const double R2 = 2.0 * params[0];
+    const double c[3] = {params[3], params[4], params[5]};
+    const int n[3] = {h->g.nx, h->g.ny, h->g.nz};
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::max(0, (int)std::floor(c[a] - R2) - 1);
+      hi[a] = std::min(n[a] - 1, (int)std::ceil(c[a] + R2) + 1);
+    }
+    if (hi[0] >= lo[0] && hi[1] >= lo[1] && hi[2] >= lo[2]) {
+      h->psr_lo = int3{lo[0], lo[1], lo[2]};
+      h->psr_n = int3{hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+      const size_t nb = (size_t)h->psr_n.x * h->psr_n.y * h->psr_n.z * 6 * h->esz;
+      if (hipMalloc(&h->psr_stage, nb) != hipSuccess) return fail(h, TFF_ENOMEM, "psr staging");
+    }
+  }
+  if (kind != h->bc_kind) invalidate_graphs(h);
+  h->bc_kind = kind;
+  return TFF_OK;
+}
+
+int tff_init_rk_step(tff_handle* h) {
+  if (!h) return TFF_EINVAL;
+  for (Vec* v : {&h->en, &h->bn})
+    if (!v->p[0]) {
+      int rc = alloc_vec(h, *v);
+      if (rc) return rc;
+    }
+  const int64_t n = h->nalloc;
+  const int tb = 256;
+  if (h->esz == 8) {
+    hipLaunchKernelGGL(k_copy3<double>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, h->stream, n,
+                       cv3<double>(h->es), mv3<double>(h->en));
+    hipLaunchKernelGGL(k_copy3<double>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, h->stream, n,
+                       cv3<double>(h->bs), mv3<double>(h->bn));
+  } else {
+    hipLaunchKernelGGL(k_copy3<float>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, h->stream, n,
+                       cv3<float>(h->es), mv3<float>(h->en));
+    hipLaunchKernelGGL(k_copy3<float>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, h->stream, n,
+                       cv3<float>(h->bs), mv3<float>(h->bn));
+  }
+  return last_launch(h, "k_copy3");
+}
+
+int tff_rk3_step(tff_handle* h, double c1, double c2, double c3) {
+  if (!h) return TFF_EINVAL;
+  if (!h->en.p[0]) return fail(h, TFF_ESTATE, "tff_rk3_step before tff_init_rk_step");
+  return dispatch<DoRk3>(h, c1, c2, c3);
+}
+
+int tff_clean_epar(tff_handle* h) {
+  if (!h) return TFF_EINVAL;
+  int rc = dispatch<DoClean>(h);
+  invalidate_graphs(h);
+  return rc;
+}
+
+int tff_exchange_fields(tff_handle* h, int exchange_b0) {
+  if (!h) return TFF_EINVAL;
+  return dispatch<DoFill>(h, exchange_b0);
+}
+
+int tff_user_bc(tff_handle* h, int step) {
+  if (!h) return TFF_EINVAL;
+  return dispatch<DoBC>(h, step);
+}
+
+int tff_step(tff_handle* h, int timestep) {
+  if (!h) return TFF_EINVAL;
+  return dispatch<DoStep>(h, timestep, 0);
+}
+
+int tff_run(tff_handle* h, int first_step, int nsteps) {
+  if (!h || nsteps < 0) return TFF_EINVAL;
+  if (nsteps == 0) return TFF_OK;
+  HIPCHK(h, hipMemcpyAsync(h->d_step, &first_step, sizeof(int), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int n = 0; n < nsteps; ++n) {
+    int rc = dispatch<DoStep>(h, first_step + n, 1);
+    if (rc) return rc;
+  }
+  return TFF_OK;
+}
+
+int tff_reduce(tff_handle* h, double out[4]) {
+  if (!h || !out) return TFF_EINVAL;
+  return dispatch<DoReduce>(h, out);
+}
+
+int tff_sync(tff_handle* h) {
+  if (!h) return TFF_EINVAL;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return TFF_OK;
+}
+
+int tff_event_record(tff_handle* h, int slot) {
+  if (!h || slot < 0 || slot >= 16) return TFF_EINVAL;
+  HIPCHK(h, hipEventRecord(h->ev[slot], h->stream));
+  return TFF_OK;
+}
+
+int tff_event_elapsed(tff_handle* h, int a, int b, float* ms) {
+  if (!h || a < 0 || a >= 16 || b < 0 || b >= 16 || !ms) return TFF_EINVAL;
+  HIPCHK(h, hipEventSynchronize(h->ev[b]));
+  HIPCHK(h, hipEventElapsedTime(ms, h->ev[a], h->ev[b]));
+  return TFF_OK;
+}
+
+int tff_set_kernel_timing(tff_handle* h, int on) {
+  if (!h) return TFF_EINVAL;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->timing = on != 0;
+  h->tused = 0;
+  h->tsum_ms = 0.0;
+  h->tcount = 0;
+  return TFF_OK;
+}
+
+int tff_stage_kernel_time(tff_handle* h, double* total_ms, long* launches) {
+  if (!h || !total_ms || !launches) return TFF_EINVAL;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (size_t t = 0; t + 1 < h->tused; t += 2) {
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->tpool[t], h->tpool[t + 1]));
+    h->tsum_ms += ms;
+    h->tcount += 1;
+  }
+  h->tused = 0;
+  *total_ms = h->tsum_ms;
+  *launches = h->tcount;
+  return TFF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,669 @@
+// tff_kernels.h -- HIP kernels of the force-free step for gfx950.
+//
+// One thread per cell of the allocated box (interior + NGHOST ghosts),
+// x fastest so that a wave64 reads 64 consecutive elements of a row.  Each
+// kernel writes every cell of its output: the interior gets the new value,
+// ghost cells get the input's ghost value, so an output buffer carries the
+// same ghost contents the reference's in-place arrays keep.
+#pragma once
+#include "tff_common.h"
+
+namespace tff {
+
+constexpr int BX = 64;  // threads along x: one wave64 per row segment
+constexpr int BY = 4;
+
+__device__ __forceinline__ bool interior(const Geom& g, int i, int j, int k) {
+  return i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
+}
+
+// ---------------------------------------------------------------------------
+// rk3Step (field_solver.F90:35-42): computeRho (:44-63), rk3Push (:65-166) and
+// rk3Update (:168-271) fused per cell.  Reads X = (xe, xb) with a one-cell
+// stencil, U0 = (ue, ub) at the cell, the precomputed B0 curl term t0 and
+// writes Y = c1*U0 + c2*X + c3*L(X) on the interior.  With P == T == S the
+// operation order is the reference's, so FP64 and FP32 results are those of
+// field_solver.F90 bit for bit (built with -ffp-contract=off).
+template <class PR>
+__global__ void __launch_bounds__(BX* BY)
+    k_rk3(Geom g, V3<const typename PR::S> xe, V3<const typename PR::S> xb,
+          V3<const typename PR::S> ue, V3<const typename PR::S> ub,
+          V3<const typename PR::S> t0, int has_t0, V3<typename PR::S> ye,
+          V3<typename PR::S> yb, typename PR::P cc, typename PR::P c1, typename PR::P c2,
+          typename PR::P c3, Absorb<typename PR::P> ab) {
+  using S = typename PR::S;
+  using T = typename PR::T;
+  using P = typename PR::P;
+  const int i = blockIdx.x * BX + threadIdx.x - NG;
+  const int j = blockIdx.y * BY + threadIdx.y - NG;
+  const int k = blockIdx.z - NG;
+  if (i > g.nx - 1 + NG || j > g.ny - 1 + NG) return;
+  const int64_t c = g.at(i, j, k);
+  if (!interior(g, i, j, k)) {
+    ye.x[c] = xe.x[c];
+    ye.y[c] = xe.y[c];
+    ye.z[c] = xe.z[c];
+    yb.x[c] = xb.x[c];
+    yb.y[c] = xb.y[c];
+    yb.z[c] = xb.z[c];
+    return;
+  }
+  const int64_t sx = 1, sy = g.px, sz = g.plane;
+#define EX(a, b, d) ((T)xe.x[c + (a)*sx + (b)*sy + (d)*sz])
+#define EY(a, b, d) ((T)xe.y[c + (a)*sx + (b)*sy + (d)*sz])
+#define EZ(a, b, d) ((T)xe.z[c + (a)*sx + (b)*sy + (d)*sz])
+#define BXF(a, b, d) ((T)xb.x[c + (a)*sx + (b)*sy + (d)*sz])
+#define BYF(a, b, d) ((T)xb.y[c + (a)*sx + (b)*sy + (d)*sz])
+#define BZF(a, b, d) ((T)xb.z[c + (a)*sx + (b)*sy + (d)*sz])
+#define RHO(a, b, d) \
+  ((EX(a, b, d) - EX(a - 1, b, d)) + (EY(a, b, d) - EY(a, b - 1, d)) + (EZ(a, b, d) - EZ(a, b, d - 1)))
+  // curl (field_solver.F90:81-89)
+  const T tx = has_t0 ? (T)t0.x[c] : (T)0;
+  const T ty = has_t0 ? (T)t0.y[c] : (T)0;
+  const T tz = has_t0 ? (T)t0.z[c] : (T)0;
+  P dbx = cc * (P)((EY(0, 0, 1) - EY(0, 0, 0)) - (EZ(0, 1, 0) - EZ(0, 0, 0)));
+  P dby = cc * (P)((EZ(1, 0, 0) - EZ(0, 0, 0)) - (EX(0, 0, 1) - EX(0, 0, 0)));
+  P dbz = cc * (P)((EX(0, 1, 0) - EX(0, 0, 0)) - (EY(1, 0, 0) - EY(0, 0, 0)));
+  P dex = -cc * (P)(((BYF(0, 0, 0) - BYF(0, 0, -1)) - (BZF(0, 0, 0) - BZF(0, -1, 0))) - tx);
+  P dey = -cc * (P)(((BZF(0, 0, 0) - BZF(-1, 0, 0)) - (BXF(0, 0, 0) - BXF(0, 0, -1))) - ty);
+  P dez = -cc * (P)(((BXF(0, 0, 0) - BXF(0, -1, 0)) - (BYF(0, 0, 0) - BYF(-1, 0, 0))) - tz);
+  // charge density at the four nodes the currents need (computeRho :57-59)
+  const T r000 = RHO(0, 0, 0);
+  const T r100 = RHO(1, 0, 0);
+  const T r010 = RHO(0, 1, 0);
+  const T r001 = RHO(0, 0, 1);
+  const T h = (T)0.5, q = (T)0.25, e = (T)0.125;
+  {  // X current (:90-111)
+    T intrho = h * (r100 + r000);
+    T intex = EX(0, 0, 0);
+    T intey = q * (EY(0, 0, 0) + EY(1, 0, 0) + EY(0, -1, 0) + EY(1, -1, 0));
+    T intez = q * (EZ(0, 0, 0) + EZ(1, 0, 0) + EZ(0, 0, -1) + EZ(1, 0, -1));
+    T intbx = e * (BXF(0, 0, 0) + BXF(0, -1, 0) + BXF(1, -1, 0) + BXF(1, 0, 0) + BXF(0, 0, -1) +
+                   BXF(0, -1, -1) + BXF(1, -1, -1) + BXF(1, 0, -1));
+    T intby = h * (BYF(0, 0, 0) + BYF(0, 0, -1));
+    T intbz = h * (BZF(0, 0, 0) + BZF(0, -1, 0));
+    dex = dex - drift_current<P>(cc, (P)intrho, (P)intex, (P)intey, (P)intez, (P)intbx, (P)intby,
+                                 (P)intbz, 0);
+  }
+  {  // Y current (:113-134)
+    T intrho = h * (r010 + r000);
+    T intex = q * (EX(0, 0, 0) + EX(-1, 0, 0) + EX(0, 1, 0) + EX(-1, 1, 0));
+    T intey = EY(0, 0, 0);
+    T intez = q * (EZ(0, 0, 0) + EZ(0, 1, 0) + EZ(0, 0, -1) + EZ(0, 1, -1));
+    T intbx = h * (BXF(0, 0, 0) + BXF(0, 0, -1));
+    T intby = e * (BYF(0, 0, 0) + BYF(-1, 0, 0) + BYF(-1, 1, 0) + BYF(0, 1, 0) + BYF(0, 0, -1) +
+                   BYF(-1, 0, -1) + BYF(-1, 1, -1) + BYF(0, 1, -1));
+    T intbz = h * (BZF(0, 0, 0) + BZF(-1, 0, 0));
+    dey = dey - drift_current<P>(cc, (P)intrho, (P)intey, (P)intex, (P)intez, (P)intby, (P)intbx,
+                                 (P)intbz, 1);
+  }
+  {  // Z current (:136-157)
+    T intrho = h * (r000 + r001);
+    T intex = q * (EX(0, 0, 0) + EX(-1, 0, 0) + EX(0, 0, 1) + EX(-1, 0, 1));
+    T intey = q * (EY(0, 0, 0) + EY(0, -1, 0) + EY(0, 0, 1) + EY(0, -1, 1));
+    T intez = EZ(0, 0, 0);
+    T intbx = h * (BXF(0, 0, 0) + BXF(0, -1, 0));
+    T intby = h * (BYF(0, 0, 0) + BYF(-1, 0, 0));
+    T intbz = e * (BZF(0, 0, 0) + BZF(-1, 0, 0) + BZF(-1, -1, 0) + BZF(0, -1, 0) + BZF(0, 0, 1) +
+                   BZF(-1, 0, 1) + BZF(-1, -1, 1) + BZF(0, -1, 1));
+    dez = dez - drift_current<P>(cc, (P)intrho, (P)intez, (P)intex, (P)intey, (P)intbz, (P)intbx,
+                                 (P)intby, 2);
+  }
+  // rk3Update (:187-193): ((c1*Un + c2*U) + c3*dU), rounded once to S
+  P nex = c1 * (P)ue.x[c] + c2 * (P)xe.x[c] + c3 * dex;
+  P ney = c1 * (P)ue.y[c] + c2 * (P)xe.y[c] + c3 * dey;
+  P nez = c1 * (P)ue.z[c] + c2 * (P)xe.z[c] + c3 * dez;
+  P nbx = c1 * (P)ub.x[c] + c2 * (P)xb.x[c] + c3 * dbx;
+  P nby = c1 * (P)ub.y[c] + c2 * (P)xb.y[c] + c3 * dby;
+  P nbz = c1 * (P)ub.z[c] + c2 * (P)xb.z[c] + c3 * dbz;
+  if (ab.on) {
+    // absorbing layer (:200-265); the reference stores first and damps the
+    // stored value, so the FP32 path rounds before damping as well
+    nex = (P)(S)nex; ney = (P)(S)ney; nez = (P)(S)nez;
+    nbx = (P)(S)nbx; nby = (P)(S)nby; nbz = (P)(S)nbz;
+    const P ig = (P)i, jg = ab.fixed ? (P)j : (P)i, kg = (P)k;
+    P r = sqrt((ig - ab.xc) * (ig - ab.xc) + (jg - ab.yc) * (jg - ab.yc) + (kg - ab.zc) * (kg - ab.zc));
+    if (r > ab.rabsorb - (P)1.0) {
+      const P hh = (P)0.5, zz = (P)0.0;
+      P d;
+#define DAMP(v, ox, oy, oz)                                                                         \
+  r = sqrt((ig + (ox)-ab.xc) * (ig + (ox)-ab.xc) + (jg + (oy)-ab.yc) * (jg + (oy)-ab.yc) +         \
+           (kg + (oz)-ab.zc) * (kg + (oz)-ab.zc));                                                 \
+  d = (r - ab.rabsorb) / ab.buff;                                                                   \
+  if (ab.fixed && d > (P)1.0) d = (P)1.0;                                                           \
+  if (d > (P)0.0) v = ((P)1.0 - d * d * d) * v;
+      DAMP(nex, hh, zz, zz)
+      DAMP(ney, zz, hh, zz)
+      DAMP(nez, zz, zz, hh)
+      DAMP(nbx, zz, hh, hh)
+      DAMP(nby, hh, zz, hh)
+      DAMP(nbz, hh, hh, zz)
+#undef DAMP
+    }
+  }
+  ye.x[c] = (S)nex;
+  ye.y[c] = (S)ney;
+  ye.z[c] = (S)nez;
+  yb.x[c] = (S)nbx;
+  yb.y[c] = (S)nby;
+  yb.z[c] = (S)nbz;
+#undef RHO
+}
+
+// ---------------------------------------------------------------------------
+// cleanEpar (field_solver.F90:273-335): E <- E - (E.B) B / |B|^2 with the
+// staggered interpolants, no TINY (an exact magnetic null gives NaN exactly
+// as the reference does).  Interior only; ghosts carry the input value.
+template <class PR>
+__global__ void __launch_bounds__(BX* BY)
+    k_clean(Geom g, V3<const typename PR::S> xe, V3<const typename PR::S> xb,
+            V3<typename PR::S> ye) {
+  using S = typename PR::S;
+  using T = typename PR::T;
+  using P = typename PR::P;
+  const int i = blockIdx.x * BX + threadIdx.x - NG;
+  const int j = blockIdx.y * BY + threadIdx.y - NG;
+  const int k = blockIdx.z - NG;
+  if (i > g.nx - 1 + NG || j > g.ny - 1 + NG) return;
+  const int64_t c = g.at(i, j, k);
+  if (!interior(g, i, j, k)) {
+    ye.x[c] = xe.x[c];
+    ye.y[c] = xe.y[c];
+    ye.z[c] = xe.z[c];
+    return;
+  }
+  const int64_t sx = 1, sy = g.px, sz = g.plane;
+  const T h = (T)0.5, q = (T)0.25, e = (T)0.125;
+  P outx, outy, outz;
+  {
+    T intex = EX(0, 0, 0);
+    T intey = q * (EY(0, 0, 0) + EY(1, 0, 0) + EY(0, -1, 0) + EY(1, -1, 0));
+    T intez = q * (EZ(0, 0, 0) + EZ(1, 0, 0) + EZ(0, 0, -1) + EZ(1, 0, -1));
+    T intbx = e * (BXF(0, 0, 0) + BXF(0, -1, 0) + BXF(1, -1, 0) + BXF(1, 0, 0) + BXF(0, 0, -1) +
+                   BXF(0, -1, -1) + BXF(1, -1, -1) + BXF(1, 0, -1));
+    T intby = h * (BYF(0, 0, 0) + BYF(0, 0, -1));
+    T intbz = h * (BZF(0, 0, 0) + BZF(0, -1, 0));
+    P bx_ = intbx, by_ = intby, bz_ = intbz;
+    P bmag = bx_ * bx_ + by_ * by_ + bz_ * bz_;
+    P epar = ((P)intex * bx_ + (P)intey * by_ + (P)intez * bz_);
+    outx = (P)xe.x[c] - epar * bx_ / bmag;
+  }
+  {
+    T intex = q * (EX(0, 0, 0) + EX(-1, 0, 0) + EX(0, 1, 0) + EX(-1, 1, 0));
+    T intey = EY(0, 0, 0);
+    T intez = q * (EZ(0, 0, 0) + EZ(0, 1, 0) + EZ(0, 0, -1) + EZ(0, 1, -1));
+    T intbx = h * (BXF(0, 0, 0) + BXF(0, 0, -1));
+    T intby = e * (BYF(0, 0, 0) + BYF(-1, 0, 0) + BYF(-1, 1, 0) + BYF(0, 1, 0) + BYF(0, 0, -1) +
+                   BYF(-1, 0, -1) + BYF(-1, 1, -1) + BYF(0, 1, -1));
+    T intbz = h * (BZF(0, 0, 0) + BZF(-1, 0, 0));
+    P bx_ = intbx, by_ = intby, bz_ = intbz;
+    P bmag = bx_ * bx_ + by_ * by_ + bz_ * bz_;
+    P epar = ((P)intex * bx_ + (P)intey * by_ + (P)intez * bz_);
+    outy = (P)xe.y[c] - epar * by_ / bmag;
+  }
+  {
+    T intex = q * (EX(0, 0, 0) + EX(-1, 0, 0) + EX(0, 0, 1) + EX(-1, 0, 1));
+    T intey = q * (EY(0, 0, 0) + EY(0, -1, 0) + EY(0, 0, 1) + EY(0, -1, 1));
+    T intez = EZ(0, 0, 0);
+    T intbx = h * (BXF(0, 0, 0) + BXF(0, -1, 0));
+    T intby = h * (BYF(0, 0, 0) + BYF(-1, 0, 0));
+    T intbz = e * (BZF(0, 0, 0) + BZF(-1, 0, 0) + BZF(-1, -1, 0) + BZF(0, -1, 0) + BZF(0, 0, 1) +
+                   BZF(-1, 0, 1) + BZF(-1, -1, 1) + BZF(0, -1, 1));
+    P bx_ = intbx, by_ = intby, bz_ = intbz;
+    P bmag = bx_ * bx_ + by_ * by_ + bz_ * bz_;
+    P epar = ((P)intex * bx_ + (P)intey * by_ + (P)intez * bz_);
+    outz = (P)xe.z[c] - epar * bz_ / bmag;
+  }
+  ye.x[c] = (S)outx;
+  ye.y[c] = (S)outy;
+  ye.z[c] = (S)outz;
+}
+#undef EX
+#undef EY
+#undef EZ
+#undef BXF
+#undef BYF
+#undef BZF
+
+// ---------------------------------------------------------------------------
+// Ghost shell enumeration: z-slabs (all i, j), then y-slabs (interior k, all
+// i), then x-slabs (interior j, k).  Every ghost cell appears exactly once.
+__device__ __forceinline__ bool shell_cell(const Geom& g, int64_t t, int& i, int& j, int& k) {
+  const int sx = g.nx + 2 * NG, sy = g.ny + 2 * NG;
+  const int64_t nzc = (int64_t)2 * NG * sx * sy;
+  const int64_t nyc = (int64_t)2 * NG * sx * g.nz;
+  const int64_t nxc = (int64_t)2 * NG * g.ny * g.nz;
+  if (t < nzc) {
+    i = (int)(t % sx) - NG;
+    t /= sx;
+    j = (int)(t % sy) - NG;
+    const int kk = (int)(t / sy);
+    k = kk < NG ? kk - NG : g.nz + kk - NG;
+    return true;
+  }
+  t -= nzc;
+  if (t < nyc) {
+    i = (int)(t % sx) - NG;
+    t /= sx;
+    k = (int)(t % g.nz);
+    const int jj = (int)(t / g.nz);
+    j = jj < NG ? jj - NG : g.ny + jj - NG;
+    return true;
+  }
+  t -= nyc;
+  if (t < nxc) {
+    const int ii = (int)(t % (2 * NG));
+    t /= (2 * NG);
+    j = (int)(t % g.ny);
+    k = (int)(t / g.ny);
+    i = ii < NG ? ii - NG : g.nx + ii - NG;
+    return true;
+  }
+  return false;
+}
+
+// exchangeFields on one meshblock (exchange_fields.F90:44-195 with every
+// neighbour being the block itself, helpers.F90:22-46): a ghost cell is
+// filled iff every axis on which it lies outside the interior is periodic,
+// and it takes the value of the wrapped interior cell.
+template <class S>
+struct FillSet {
+  S* a[9];
+  int n;
+};
+
+template <class S>
+__global__ void k_fill(Geom g, int px_, int py_, int pz_, FillSet<S> fs, int64_t nshell) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nshell) return;
+  int i, j, k;
+  shell_cell(g, t, i, j, k);
+  const bool ox = i < 0 || i >= g.nx, oy = j < 0 || j >= g.ny, oz = k < 0 || k >= g.nz;
+  if ((ox && !px_) || (oy && !py_) || (oz && !pz_)) return;
+  const int wi = i < 0 ? i + g.nx : (i >= g.nx ? i - g.nx : i);
+  const int wj = j < 0 ? j + g.ny : (j >= g.ny ? j - g.ny : j);
+  const int wk = k < 0 ? k + g.nz : (k >= g.nz ? k - g.nz : k);
+  const int64_t d = g.at(i, j, k), s = g.at(wi, wj, wk);
+  for (int f = 0; f < fs.n; ++f) fs.a[f][d] = fs.a[f][s];
+}
+
+// Curl of the background field, the B0 bracket of rk3Push (:84-89),
+// evaluated once: t0 = ((b0y(k)-b0y(k-1)) - (b0z(j)-b0z(j-1))) and cyclic.
+template <class S>
+__global__ void __launch_bounds__(BX* BY)
+    k_b0_curl(Geom g, V3<const S> b0, V3<S> t0, int* nonzero) {
+  const int i = blockIdx.x * BX + threadIdx.x;
+  const int j = blockIdx.y * BY + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  const int64_t c = g.at(i, j, k), sy = g.px, sz = g.plane;
+  const S tx = (b0.y[c] - b0.y[c - sz]) - (b0.z[c] - b0.z[c - sy]);
+  const S ty = (b0.z[c] - b0.z[c - 1]) - (b0.x[c] - b0.x[c - sz]);
+  const S tz = (b0.x[c] - b0.x[c - sy]) - (b0.y[c] - b0.y[c - 1]);
+  t0.x[c] = tx;
+  t0.y[c] = ty;
+  t0.z[c] = tz;
+  if (tx != (S)0 || ty != (S)0 || tz != (S)0) atomicOr(nonzero, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Boundary hooks.
+
+// user_loop.F90:181-244.  p: 0 xc, 1 yc1, 2 yc2, 3 zc, 4 bnorm, 5 r0,
+// 6 beta_twist, 7 t0, 8 twist flag (0: floor/top copy as shipped; 1: the
+// footpoint-rotation driver of the commented block :202-230 with its
+// tanh(time/t0) ramp on).  Only z-ghost planes are written.
+template <class PR>
+__global__ void k_bc_loop(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, const double* p,
+                          typename PR::P cc, const int* step_ptr, int step_val) {
+  using S = typename PR::S;
+  using P = typename PR::P;
+  const int sx = g.nx + 2 * NG, sy = g.ny + 2 * NG;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)2 * NG * sx * sy) return;
+  const int i = (int)(t % sx) - NG;
+  const int j = (int)((t / sx) % sy) - NG;
+  const int kk = (int)(t / ((int64_t)sx * sy));
+  const int k = kk < NG ? kk - NG : g.nz + kk - NG;
+  const int64_t c = g.at(i, j, k);
+  if (k < 0) {
+    if (p[8] != 0.0) {
+      const int step = step_ptr ? *step_ptr : step_val;
+      const P xc = (P)p[0], yc1 = (P)p[1], yc2 = (P)p[2], zc = (P)p[3], bnorm = (P)p[4];
+      const P r0 = (P)p[5], beta = (P)p[6], t0 = (P)p[7];
+      const P xg = (P)i, yg = (P)j, zg = (P)k, time = (P)step, hf = (P)0.5;
+      P rx, ry1, ry2, rz, rr1, rr2, bzl, omega, ee0;
+      rx = xg + hf - xc; ry1 = yg - yc1; ry2 = yg - yc2; rz = zg - zc;
+      rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
+      rr2 = sqrt(rx * rx + ry2 * ry2 + rz * rz);
+      bzl = bnorm * (rz / (rr1 * rr1 * rr1) - rz / (rr2 * rr2 * rr2));
+      omega = exp(-rr1 / r0) * beta * cc / r0;
+      ee0 = -omega * bzl * tanh(time / t0);
+      e.x[c] = (S)(rx * ee0);
+      rx = xg - xc; ry1 = yg + hf - yc1; ry2 = yg + hf - yc2; rz = zg - zc;
+      rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
+      rr2 = sqrt(rx * rx + ry2 * ry2 + rz * rz);
+      bzl = bnorm * (rz / (rr1 * rr1 * rr1) - rz / (rr2 * rr2 * rr2));
+      omega = exp(-rr1 / r0) * beta * cc / r0;
+      ee0 = -omega * bzl * tanh(time / t0);
+      e.y[c] = (S)(ry1 * ee0);
+      rx = xg + hf - xc; ry1 = yg + hf - yc1; ry2 = yg + hf - yc2; rz = zg - zc;
+      rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
+      rr2 = sqrt(rx * rx + ry2 * ry2 + rz * rz);
+      b.z[c] = (S)(bnorm * (rz / (rr1 * rr1 * rr1) - rz / (rr2 * rr2 * rr2)));
+    } else {
+      const int64_t s = g.at(i, j, 0);
+      e.x[c] = e.x[s];
+      e.y[c] = e.y[s];
+      b.z[c] = b.z[s];
+    }
+  } else {
+    const int64_t s = g.at(i, j, g.nz - 1);
+    e.x[c] = e.x[s];
+    e.y[c] = e.y[s];
+    b.z[c] = b.z[s];
+  }
+}
+
+// user_uniform.F90:77-147.  p: 0 xc, 1 yc, 2 r0 (= r_L0*L0), 3 beta_twist,
+// 4 t0, 5 bnorm.  Writes the planes k <= 0 (the interior plane k = 0
+// included, as z_glob .le. 0 does) and the top ghost planes.
+template <class PR>
+__global__ void k_bc_uniform(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, const double* p,
+                             typename PR::P cc, const int* step_ptr, int step_val) {
+  using S = typename PR::S;
+  using P = typename PR::P;
+  const int sx = g.nx + 2 * NG, sy = g.ny + 2 * NG;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)(2 * NG + 1) * sx * sy) return;
+  const int i = (int)(t % sx) - NG;
+  const int j = (int)((t / sx) % sy) - NG;
+  const int kk = (int)(t / ((int64_t)sx * sy));
+  const int k = kk <= NG ? kk - NG : g.nz + kk - NG - 1;
+  const int64_t c = g.at(i, j, k);
+  const int step = step_ptr ? *step_ptr : step_val;
+  const P xc = (P)p[0], yc = (P)p[1], r0 = (P)p[2], beta = (P)p[3], t0 = (P)p[4], bnorm = (P)p[5];
+  const P xg = (P)i, yg = (P)j, time = (P)step, hf = (P)0.5, tiny = (P)1e-7, one = (P)1.0;
+  P rx, ry, rr, omega, ee0;
+  if (k <= 0) {
+    rx = xg + hf - xc; ry = yg - yc;
+    rr = sqrt(rx * rx + ry * ry + tiny);
+    omega = exp(-rr / r0) * beta * cc / r0;
+    ee0 = -omega * bnorm * tanh(time / t0);
+    e.x[c] = (S)(rx * ee0);
+    rx = xg - xc; ry = yg + hf - yc;
+    rr = sqrt(rx * rx + ry * ry + tiny);
+    omega = exp(-rr / r0) * beta * cc / r0;
+    ee0 = -omega * bnorm * tanh(time / t0);
+    e.y[c] = (S)(ry * ee0);
+    rx = xg + hf - xc; ry = yg + hf - yc;
+    rr = sqrt(rx * rx + ry * ry + tiny);
+    b.z[c] = (S)(bnorm / (one + (rr / r0)));
+  } else {
+    rx = xg + hf - xc; ry = yg + hf - yc;
+    rr = sqrt(rx * rx + ry * ry + tiny);
+    b.z[c] = (S)(bnorm / (one + (rr / r0)));
+    e.x[c] = (S)0;
+    e.y[c] = (S)0;
+  }
+}
+
+// user_psr.F90:345-374 getDipole (p: 0 psr_radius, 1 psr_angle [rad],
+// 2 psr_omega, 3 xc_g, 4 yc_g, 5 zc_g)
+template <class P>
+__device__ __forceinline__ void dipole(const double* p, int step, P xg, P yg, P zg, P& obx, P& oby,
+                                       P& obz) {
+  const P R = (P)p[0], ang = (P)p[1], om = (P)p[2], xc = (P)p[3], yc = (P)p[4], zc = (P)p[5];
+  const P phase = om * (P)step;
+  P nx = xg - xc, ny = yg - yc, nz = zg - zc;
+  P rr = (P)1.0 / sqrt(nx * nx + ny * ny + nz * nz);
+  nx = nx * rr; ny = ny * rr; nz = nz * rr;
+  rr = rr * rr * rr;
+  const P r3 = R * R * R, off = (P)0.0;
+  const P mux = r3 * sin(ang) * cos(phase + off);
+  const P muy = r3 * sin(ang) * sin(phase + off);
+  const P muz = r3 * cos(ang);
+  const P mdn = mux * nx + muy * ny + muz * nz;
+  obx = ((P)3.0 * nx * mdn - mux) * rr;
+  oby = ((P)3.0 * ny * mdn - muy) * rr;
+  obz = ((P)3.0 * nz * mdn - muz) * rr;
+}
+
+// user_psr.F90:376-383 shape
+template <class P>
+__device__ __forceinline__ P psr_shape(P rad, P rad0) {
+  const P del = (P)1.0;
+  return (P)0.5 * ((P)1.0 - tanh((rad - rad0) / del));
+}
+
+// user_psr.F90:147-302: conducting-sphere blend for interior cells inside
+// the supersphere r < 2 R, evaluated on the box [lo, lo+n) and written to
+// the staging arrays `nw` (6 components, box layout); k_bc_psr_commit then
+// copies them back (:305-325), so every cell reads pre-BC neighbours.
+template <class PR>
+__global__ void k_bc_psr_eval(Geom g, V3<const typename PR::S> e, V3<const typename PR::S> b,
+                              const double* p, typename PR::P ccinv, int3 lo, int3 n,
+                              typename PR::S* nw, const int* step_ptr, int step_val) {
+  using S = typename PR::S;
+  using T = typename PR::T;
+  using P = typename PR::P;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nbox = (int64_t)n.x * n.y * n.z;
+  if (t >= nbox) return;
+  const int i = lo.x + (int)(t % n.x);
+  const int j = lo.y + (int)((t / n.x) % n.y);
+  const int k = lo.z + (int)(t / ((int64_t)n.x * n.y));
+  const P R = (P)p[0], om = (P)p[2], xc = (P)p[3], yc = (P)p[4], zc = (P)p[5];
+  const P ssq = (R * (P)2) * (R * (P)2);
+  const P fi = (P)i, fj = (P)j, fk = (P)k;
+  if (!((fi - xc) * (fi - xc) + (fj - yc) * (fj - yc) + (fk - zc) * (fk - zc) < ssq)) return;
+  const int step = step_ptr ? *step_ptr : step_val;
+  const int64_t c = g.at(i, j, k), sy = g.px, sz = g.plane;
+#define A(arr, a, bb, d) ((P)(T)arr[c + (a) + (bb)*sy + (d)*sz])
+  const P shift_B = (P)4.0, shift_E = (P)0.0, one = (P)1.0, hf = (P)0.5, qq = (P)0.25;
+  const P sEpar = (P)0.5, sEperp = (P)0.25, sBperp = (P)0.5, sBpar = (P)0.25;
+  P rx, ry, rz, rr_sqr, s, scale, bdx, bdy, bdz, bidr, bddr, v;
+  P out[6];
+  // B_x (:157-174)
+  rx = fi - xc; ry = fj + hf - yc; rz = fk + hf - zc;
+  rr_sqr = rx * rx + ry * ry + rz * rz;
+  bidr = A(b.x, 0, 0, 0) * rx + qq * (A(b.y, 0, 0, 0) + A(b.y, 0, 1, 0) + A(b.y, -1, 0, 0) + A(b.y, -1, 1, 0)) * ry +
+         qq * (A(b.z, 0, 0, 0) + A(b.z, 0, 0, 1) + A(b.z, -1, 0, 0) + A(b.z, -1, 0, 1)) * rz;
+  dipole<P>(p, step, rx + xc, ry + yc, rz + zc, bdx, bdy, bdz);
+  bddr = bdx * rx + bdy * ry + bdz * rz;
+  scale = one / sBperp;
+  s = psr_shape<P>(sqrt(rr_sqr) * scale, (R - shift_B) * scale);
+  v = (bidr - bddr) * (rx / rr_sqr) * (one - s);
+  scale = one / sBpar;
+  s = psr_shape<P>(sqrt(rr_sqr) * scale, (R - shift_B) * scale);
+  out[3] = v + bdx + ((A(b.x, 0, 0, 0) - bidr * rx / rr_sqr) - (bdx - bddr * rx / rr_sqr)) * (one - s);
+  // B_y (:177-194)
+  rx = fi + hf - xc; ry = fj - yc; rz = fk + hf - zc;
+  rr_sqr = rx * rx + ry * ry + rz * rz;
+  bidr = qq * (A(b.x, 0, 0, 0) + A(b.x, 1, 0, 0) + A(b.x, 0, -1, 0) + A(b.x, 1, -1, 0)) * rx + A(b.y, 0, 0, 0) * ry +
+         qq * (A(b.z, 0, 0, 0) + A(b.z, 0, 0, 1) + A(b.z, 0, -1, 0) + A(b.z, 0, -1, 1)) * rz;
+  dipole<P>(p, step, rx + xc, ry + yc, rz + zc, bdx, bdy, bdz);
+  bddr = bdx * rx + bdy * ry + bdz * rz;
+  scale = one / sBperp;
+  s = psr_shape<P>(sqrt(rr_sqr) * scale, (R - shift_B) * scale);
+  v = (bidr - bddr) * (ry / rr_sqr) * (one - s);
+  scale = one / sBpar;
+  s = psr_shape<P>(sqrt(rr_sqr) * scale, (R - shift_B) * scale);
+  out[4] = v + bdy + ((A(b.y, 0, 0, 0) - bidr * ry / rr_sqr) - (bdy - bddr * ry / rr_sqr)) * (one - s);
+  // B_z (:197-214)
+  rx = fi + hf - xc; ry = fj + hf - yc; rz = fk - zc;
+  rr_sqr = rx * rx + ry * ry + rz * rz;
+  bidr = qq * (A(b.x, 0, 0, 0) + A(b.x, 1, 0, 0) + A(b.x, 0, 0, -1) + A(b.x, 1, 0, -1)) * rx +
+         qq * (A(b.y, 0, 0, 0) + A(b.y, 0, 1, 0) + A(b.y, 0, 0, -1) + A(b.y, 0, 1, -1)) * ry + A(b.z, 0, 0, 0) * rz;
+  dipole<P>(p, step, rx + xc, ry + yc, rz + zc, bdx, bdy, bdz);
+  bddr = bdx * rx + bdy * ry + bdz * rz;
+  scale = one / sBperp;
+  s = psr_shape<P>(sqrt(rr_sqr) * scale, (R - shift_B) * scale);
+  v = (bidr - bddr) * (rz / rr_sqr) * (one - s);
+  scale = one / sBpar;
+  s = psr_shape<P>(sqrt(rr_sqr) * scale, (R - shift_B) * scale);
+  out[5] = v + bdz + ((A(b.z, 0, 0, 0) - bidr * rz / rr_sqr) - (bdz - bddr * rz / rr_sqr)) * (one - s);
+
+  P vx, vy, vz, edx, edy, edz, eidr, eddr;
+  // E_x (:219-243)
+  rx = fi + hf - xc; ry = fj - yc; rz = fk - zc;
+  rr_sqr = rx * rx + ry * ry + rz * rz;
+  eidr = A(e.x, 0, 0, 0) * rx + qq * (A(e.y, 0, 0, 0) + A(e.y, 1, 0, 0) + A(e.y, 0, -1, 0) + A(e.y, 1, -1, 0)) * ry +
+         qq * (A(e.z, 0, 0, 0) + A(e.z, 1, 0, 0) + A(e.z, 0, 0, -1) + A(e.z, 1, 0, -1)) * rz;
+  dipole<P>(p, step, rx + xc, ry + yc, rz + zc, bdx, bdy, bdz);
+  vx = -om * ry; vy = om * rx; vz = (P)0.0;
+  edx = -(vy * bdz - vz * bdy) * ccinv;
+  edy = (vx * bdz - vz * bdx) * ccinv;
+  edz = -(vx * bdy - vy * bdx) * ccinv;
+  eddr = edx * rx + edy * ry + edz * rz;
+  scale = one / sEpar;
+  s = psr_shape<P>(sqrt(rr_sqr) * scale, (R - shift_E) * scale);
+  v = edx + ((A(e.x, 0, 0, 0) - eidr * rx / rr_sqr) - (edx - eddr * rx / rr_sqr)) * (one - s);
+  scale = one / sEperp;
+  s = psr_shape<P>(sqrt(rr_sqr) * scale, (R - shift_E) * scale);
+  out[0] = v + (eidr - eddr) * (rx / rr_sqr) * (one - s);
+  // E_y (:246-270)
+  rx = fi - xc; ry = fj + hf - yc; rz = fk - zc;
+  rr_sqr = rx * rx + ry * ry + rz * rz;
+  eidr = qq * (A(e.x, 0, 0, 0) + A(e.x, 0, 1, 0) + A(e.x, -1, 0, 0) + A(e.x, -1, 1, 0)) * rx + A(e.y, 0, 0, 0) * ry +
+         qq * (A(e.z, 0, 0, 0) + A(e.z, 0, 1, 0) + A(e.z, 0, 0, -1) + A(e.z, 0, 1, -1)) * rz;
+  dipole<P>(p, step, rx + xc, ry + yc, rz + zc, bdx, bdy, bdz);
+  vx = -om * ry; vy = om * rx; vz = (P)0.0;
+  edx = -(vy * bdz - vz * bdy) * ccinv;
+  edy = (vx * bdz - vz * bdx) * ccinv;
+  edz = -(vx * bdy - vy * bdx) * ccinv;
+  eddr = edx * rx + edy * ry + edz * rz;
+  scale = one / sEpar;
+  s = psr_shape<P>(sqrt(rr_sqr) * scale, (R - shift_E) * scale);
+  v = edy + ((A(e.y, 0, 0, 0) - eidr * ry / rr_sqr) - (edy - eddr * ry / rr_sqr)) * (one - s);
+  scale = one / sEperp;
+  s = psr_shape<P>(sqrt(rr_sqr) * scale, (R - shift_E) * scale);
+  out[1] = v + (eidr - eddr) * (ry / rr_sqr) * (one - s);
+  // E_z (:273-297)
+  rx = fi - xc; ry = fj - yc; rz = fk + hf - zc;
+  rr_sqr = rx * rx + ry * ry + rz * rz;
+  eidr = qq * (A(e.x, 0, 0, 0) + A(e.x, 0, 0, 1) + A(e.x, -1, 0, 0) + A(e.x, -1, 0, 1)) * rx +
+         qq * (A(e.y, 0, 0, 0) + A(e.y, 0, -1, 0) + A(e.y, 0, 0, 1) + A(e.y, 0, -1, 1)) * ry + A(e.z, 0, 0, 0) * rz;
+  dipole<P>(p, step, rx + xc, ry + yc, rz + zc, bdx, bdy, bdz);
+  vx = -om * ry; vy = om * rx; vz = (P)0.0;
+  edx = -(vy * bdz - vz * bdy) * ccinv;
+  edy = (vx * bdz - vz * bdx) * ccinv;
+  edz = -(vx * bdy - vy * bdx) * ccinv;
+  eddr = edx * rx + edy * ry + edz * rz;
+  scale = one / sEpar;
+  s = psr_shape<P>(sqrt(rr_sqr) * scale, (R - shift_E) * scale);
+  v = edz + ((A(e.z, 0, 0, 0) - eidr * rz / rr_sqr) - (edz - eddr * rz / rr_sqr)) * (one - s);
+  scale = one / sEperp;
+  s = psr_shape<P>(sqrt(rr_sqr) * scale, (R - shift_E) * scale);
+  out[2] = v + (eidr - eddr) * (rz / rr_sqr) * (one - s);
+#undef A
+  for (int f = 0; f < 6; ++f) nw[f * nbox + t] = (S)out[f];
+}
+
+template <class PR>
+__global__ void k_bc_psr_commit(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, const double* p,
+                                int3 lo, int3 n, const typename PR::S* nw) {
+  using P = typename PR::P;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nbox = (int64_t)n.x * n.y * n.z;
+  if (t >= nbox) return;
+  const int i = lo.x + (int)(t % n.x);
+  const int j = lo.y + (int)((t / n.x) % n.y);
+  const int k = lo.z + (int)(t / ((int64_t)n.x * n.y));
+  const P R = (P)p[0], xc = (P)p[3], yc = (P)p[4], zc = (P)p[5];
+  const P ssq = (R * (P)2) * (R * (P)2);
+  const P fi = (P)i, fj = (P)j, fk = (P)k;
+  if (!((fi - xc) * (fi - xc) + (fj - yc) * (fj - yc) + (fk - zc) * (fk - zc) < ssq)) return;
+  const int64_t c = g.at(i, j, k);
+  e.x[c] = nw[0 * nbox + t];
+  e.y[c] = nw[1 * nbox + t];
+  e.z[c] = nw[2 * nbox + t];
+  b.x[c] = nw[3 * nbox + t];
+  b.y[c] = nw[4 * nbox + t];
+  b.z[c] = nw[5 * nbox + t];
+}
+
+// ---------------------------------------------------------------------------
+// FP64 diagnostics: per-block partial sums of E^2, B^2, (div_h B)^2 and the
+// block maximum of |div_h B| (wave64 shuffles, then LDS across waves); a
+// single-block pass combines the partials in a fixed order, so the result
+// does not depend on scheduling.
+constexpr int RED_THREADS = 256;
+
+__device__ __forceinline__ void wave_reduce(double& a, double& b, double& c, double& m) {
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+    c += __shfl_down(c, off, 64);
+    m = fmax(m, __shfl_down(m, off, 64));
+  }
+}
+
+template <class S>
+__global__ void __launch_bounds__(RED_THREADS)
+    k_reduce_partial(Geom g, V3<const S> e, V3<const S> b, double* part) {
+  const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
+  double se = 0.0, sb = 0.0, sd = 0.0, md = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t % g.nx);
+    const int j = (int)((t / g.nx) % g.ny);
+    const int k = (int)(t / ((int64_t)g.nx * g.ny));
+    const int64_t c = g.at(i, j, k);
+    const double ax = e.x[c], ay = e.y[c], az = e.z[c];
+    const double bx = b.x[c], by = b.y[c], bz = b.z[c];
+    se += ax * ax + ay * ay + az * az;
+    sb += bx * bx + by * by + bz * bz;
+    const double dv = ((double)b.x[c + 1] - bx) + ((double)b.y[c + g.px] - by) +
+                      ((double)b.z[c + g.plane] - bz);
+    sd += dv * dv;
+    md = fmax(md, fabs(dv));
+  }
+  wave_reduce(se, sb, sd, md);
+  __shared__ double sh[4][RED_THREADS / 64];
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  if (l == 0) {
+    sh[0][w] = se; sh[1][w] = sb; sh[2][w] = sd; sh[3][w] = md;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, bb = 0, cc = 0, m = 0;
+    for (int q = 0; q < RED_THREADS / 64; ++q) {
+      a += sh[0][q]; bb += sh[1][q]; cc += sh[2][q]; m = fmax(m, sh[3][q]);
+    }
+    part[4 * blockIdx.x + 0] = a;
+    part[4 * blockIdx.x + 1] = bb;
+    part[4 * blockIdx.x + 2] = cc;
+    part[4 * blockIdx.x + 3] = m;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_reduce_final(const double* part, int nblocks, double* out,
+                                                     double ncell) {
+  double a = 0, b = 0, c = 0, m = 0;
+  for (int q = threadIdx.x; q < nblocks; q += 64) {
+    a += part[4 * q]; b += part[4 * q + 1]; c += part[4 * q + 2]; m = fmax(m, part[4 * q + 3]);
+  }
+  wave_reduce(a, b, c, m);
+  if (threadIdx.x == 0) {
+    out[0] = 0.5 * a;
+    out[1] = 0.5 * b;
+    out[2] = m;
+    out[3] = sqrt(c / ncell);
+  }
+}
+
+__global__ void k_bump_step(int* step) {
+  if (threadIdx.x == 0) *step += 1;
+}
+
+template <class S>
+__global__ void k_copy3(int64_t n, V3<const S> a, V3<S> b) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  b.x[t] = a.x[t];
+  b.y[t] = a.y[t];
+  b.z[t] = a.z[t];
+}
+
+}  // namespace tff
