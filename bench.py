@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark of the force-free step on one MI355X.
+
+Workload: uniform B0 = z^ plus a seeded Alfvenic perturbation (|dB|/B0 = 0.05,
+1 <= |k| <= 8), 256^3 periodic Yee grid, CC = 0.45; each timed step is one
+full SSP-RK3 timestep (three stages, each with the force-free current, the
+update, the E.B = 0 projection and the ghost fill), inputs resident in HBM.
+Both paths run the same steps from the same initial state:
+
+* fp32_bulk -- FP32 storage and stencils, FP64 normalisation/projection,
+  stage accumulation and reductions (the headline ``value``);
+* fp64      -- FP64 throughout (parity-checked against the FP64 oracle).
+
+The FP32-bulk energy is compared with the FP64 path's energy after the run.
+``roofline`` is for the dominant kernel (the stage kernel of the FP32-bulk
+path): algorithmic bytes per launch over its average launch time, the time
+taken with HIP events on the library's own stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "tristan-ff_amd")
+sys.path.insert(0, PKG)
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+METRIC = "cell-updates/s + %HBM roofline: FP32-bulk vs FP64 at ≤1e-5 rel. energy err"
+
+
+def algorithmic_bytes_per_cell_update(word: int, g: int) -> int:
+    """w * (48 + 9 g): stage 1 reads U^n and writes U^1 (12 words), stages 2
+    and 3 read U^n and U^(s-1) and write U^s (18 words each), plus 3 words of
+    curl B0 per stage when B0 is non-uniform."""
+    return word * (48 + 9 * g)
+
+
+def run_path(precision: str, prob, steps: int, warmup: int, timing_steps: int):
+    import pytff
+    import problems
+    mb = pytff.Meshblock(prob.n, boundary=prob.boundary, cc=prob.cc, precision=precision,
+                         absorb_buff=prob.absorb_buff, absorb_fixed=prob.absorb_fixed)
+    problems.load(mb, prob)
+    mb.preloop()
+    e_start = mb.reduce()
+    mb.run(0, warmup)
+    mb.sync()
+    mb.event(0)
+    mb.run(warmup, steps)
+    mb.event(1)
+    ms = mb.elapsed_ms(0, 1)
+    e_end = mb.reduce()
+    # dominant-kernel timing: the same steps again with every stage-kernel
+    # launch bracketed by events on the library's stream
+    mb.kernel_timing(True)
+    for s in range(timing_steps):
+        mb.step(warmup + steps + s)
+    ktot, klaunch = mb.stage_kernel_time()
+    mb.kernel_timing(False)
+    cells = prob.cells
+    word = 8 if precision == "fp64" else 4
+    b_alg = algorithmic_bytes_per_cell_update(word, prob.g)
+    cups = cells * steps / (ms / 1e3)
+    avg_launch_s = ktot / 1e3 / max(klaunch, 1)
+    # the three stage launches move 12 + 18 + 18 words per cell: 16 on average
+    per_launch_bytes = cells * word * (48 + 9 * prob.g) / 3.0
+    achieved = per_launch_bytes / avg_launch_s
+    mb.close()
+    return {
+        "precision": precision,
+        "value": cups,
+        "ms_per_step": ms / steps,
+        "b_alg_per_cell_update": b_alg,
+        "roofline_frac_canonical": cups * b_alg / HBM_PEAK,
+        "stage_kernel": {"avg_ms": avg_launch_s * 1e3, "launches": klaunch,
+                         "alg_bytes_per_launch": per_launch_bytes,
+                         "achieved_GBps": achieved / 1e9},
+        "energy_end": e_end[0] + e_end[1],
+        "energy_start": e_start[0] + e_start[1],
+        "divb_max_end": e_end[2],
+    }
+
+
+def cpu_baseline(n: int, budget_s: float):
+    """The reference's own field_solver.F90 (oracle/_ref, FP32 as shipped,
+    amdflang -O3) on one host core: one timestep of the same workload."""
+    try:
+        import numpy as np
+        from oracle.oracle import RefSolver
+        import problems
+        if not RefSolver.available(np.float32):
+            raise FileNotFoundError("oracle/_ref not built")
+        p = problems.alfven(n, seed=1, kmax=8)
+        r = RefSolver(p.n, boundary=p.boundary, cc=p.cc, dtype=np.float32)
+        for f, a in p.fields.items():
+            r.put(f, a.astype(np.float32))
+        r.exchange()
+        t = time.time()
+        r.run(1)
+        el = time.time() - t
+        return {"value": p.cells / el, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+                "sample": f"1 timestep of the {n}^3 Alfvenic workload, reference "
+                          f"field_solver.F90 built FP32 as shipped (amdflang -O3), "
+                          f"single rank, serial periodic ghost fill; {el:.1f} s"}
+    except Exception as e:  # the baseline is reported, never required
+        return {"value": None, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+                "sample": f"unavailable: {e}"}
+
+
+def load_traffic(workload: str):
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if os.path.exists(path):
+        with open(path) as fh:
+            return json.load(fh)
+    return None
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--timing-steps", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+              "multi-GPU sharding of the meshblock is a design note, not a measured path.",
+              file=sys.stderr)
+        return 2
+    import problems
+    t0 = time.time()
+    prob = problems.alfven(a.n, seed=1, kmax=8)
+    print(f"[bench] initial condition {a.n}^3 built in {time.time() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+    res = {}
+    for prec in ("fp64", "fp32_bulk"):
+        res[prec] = run_path(prec, prob, a.steps, a.warmup, a.timing_steps)
+        print(f"[bench] {prec}: {res[prec]['value']:.4g} cell-updates/s, "
+              f"{res[prec]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    e64, e32 = res["fp64"]["energy_end"], res["fp32_bulk"]["energy_end"]
+    e_rel = abs(e32 - e64) / abs(e64)
+    de64 = e64 - res["fp64"]["energy_start"]
+    de32 = e32 - res["fp32_bulk"]["energy_start"]
+    de_rel = abs(de32 - de64) / max(abs(de64), 1e-300)
+    head = res["fp32_bulk"]
+    workload = f"alfven{a.n}"
+    roof = {
+        "bound": "hbm",
+        "achieved": head["stage_kernel"]["achieved_GBps"],
+        "peak": HBM_PEAK / 1e9,
+        "unit": "GB/s",
+        "frac": head["stage_kernel"]["achieved_GBps"] / (HBM_PEAK / 1e9),
+        "traffic": None,
+        "kernel": "k_rk3<PrecF32Bulk> (stage kernel)",
+        "avg_launch_ms": head["stage_kernel"]["avg_ms"],
+        "alg_bytes_per_launch": head["stage_kernel"]["alg_bytes_per_launch"],
+    }
+    tr = load_traffic(workload + "_fp32_bulk")
+    if tr:
+        roof["traffic"] = tr.get("bytes_per_launch")
+    cpu = None if a.no_cpu_baseline else cpu_baseline(a.n, 30.0)
+    line = {
+        "metric": METRIC,
+        "value": head["value"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 storage/stencils, f64 normalisation+accumulation (fp32_bulk)",
+        "data": "synthetic",
+        "config": {"workload": f"{workload}: B0=z + Alfvenic dB/B0=0.05 (k<=8, seed 1), "
+                               f"{a.n}^3 periodic, CC=0.45, SSP-RK3 force-free step",
+                   "cells": prob.cells, "g": prob.g, "parallelism": "single GPU"},
+        "roofline": roof,
+        "roofline_frac_canonical": head["roofline_frac_canonical"],
+        "fp64": {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
+                 "roofline_frac_canonical": res["fp64"]["roofline_frac_canonical"],
+                 "stage_kernel": res["fp64"]["stage_kernel"]},
+        "fp32_bulk_over_fp64": head["value"] / res["fp64"]["value"],
+        "energy_rel_err_fp32_bulk_vs_fp64": e_rel,
+        "delta_energy_rel_err_fp32_bulk_vs_fp64": de_rel,
+        "divb_max": {"fp64": res["fp64"]["divb_max_end"], "fp32_bulk": head["divb_max_end"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

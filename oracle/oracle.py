@@ -186,6 +186,7 @@ class RefSolver:
             lib.ref_xfer.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
             lib.ref_rk3_step.argtypes = [ctypes.c_double] * 3
             lib.ref_real_size.restype = ctypes.c_int
+            lib.ref_run.argtypes = [ctypes.c_int]
             RefSolver._libs[name] = lib
         self.lib = RefSolver._libs[name]
         assert self.lib.ref_real_size() == self.dtype.itemsize
@@ -215,3 +216,10 @@ class RefSolver:
 
     def clean_epar(self):
         self.lib.ref_clean_epar()
+
+    def exchange(self):
+        self.lib.ref_exchange()
+
+    def run(self, nsteps: int):
+        """nsteps timesteps of tristanmainloop.F90:50-83 (no user hook)."""
+        self.lib.ref_run(nsteps)

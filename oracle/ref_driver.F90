@@ -117,4 +117,47 @@ contains
   subroutine ref_clean_epar() bind(C, name='ref_clean_epar')
     call cleanEpar()
   end subroutine ref_clean_epar
+
+  ! Harness ghost fill for one meshblock (the reference's MPI exchange,
+  ! src/logistics/exchange_fields.F90, cannot be built here): a ghost cell
+  ! whose out-of-range axes are all periodic takes the wrapped interior value.
+  subroutine ref_exchange() bind(C, name='ref_exchange')
+    integer :: i, j, k, si, sj, sk, nx, ny, nz
+    logical :: ox, oy, oz
+    nx = this_meshblock%ptr%sx; ny = this_meshblock%ptr%sy; nz = this_meshblock%ptr%sz
+    do k = -NGHOST, nz - 1 + NGHOST
+      oz = (k < 0) .or. (k >= nz)
+      if (oz .and. boundary_z /= 1) cycle
+      sk = modulo(k, nz)
+      do j = -NGHOST, ny - 1 + NGHOST
+        oy = (j < 0) .or. (j >= ny)
+        if (oy .and. boundary_y /= 1) cycle
+        sj = modulo(j, ny)
+        do i = -NGHOST, nx - 1 + NGHOST
+          ox = (i < 0) .or. (i >= nx)
+          if (ox .and. boundary_x /= 1) cycle
+          if (.not. (ox .or. oy .or. oz)) cycle
+          si = modulo(i, nx)
+          ex(i, j, k) = ex(si, sj, sk); ey(i, j, k) = ey(si, sj, sk); ez(i, j, k) = ez(si, sj, sk)
+          bx(i, j, k) = bx(si, sj, sk); by(i, j, k) = by(si, sj, sk); bz(i, j, k) = bz(si, sj, sk)
+        end do
+      end do
+    end do
+  end subroutine ref_exchange
+
+  ! nsteps timesteps in the order of src/tristanmainloop.F90:50-83 for a
+  ! problem without a user boundary hook (used to time the reference).
+  subroutine ref_run(nsteps) bind(C, name='ref_run')
+    integer(c_int), value :: nsteps
+    integer :: n
+    do n = 1, nsteps
+      call initRKstep()
+      call rk3Step(1.0, 0.0, 1.0)
+      call ref_exchange(); call cleanEpar(); call ref_exchange()
+      call rk3Step(3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0)
+      call ref_exchange(); call cleanEpar(); call ref_exchange()
+      call rk3Step(1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0)
+      call ref_exchange(); call cleanEpar(); call ref_exchange()
+    end do
+  end subroutine ref_run
 end module m_refdriver

@@ -27,3 +27,17 @@ def random_fields(shape, rng, dtype=np.float64, b_offset=1.0):
             a = 0.3 * a
         out[f] = a.astype(dtype)
     return out
+
+
+def field_errors(get, ref):
+    """Normwise max-relative error of E and of B as vector fields:
+    max_c max|U_c - U_ref,c| / max_c max|U_ref,c|.  (A component that is zero
+    in exact arithmetic holds only roundoff, so normalising it by itself
+    measures nothing but the rounding of transcendental functions.)"""
+    out = {}
+    for v in ("e", "b"):
+        num = max(np.max(np.abs(np.asarray(get(v + c), np.float64) - np.asarray(ref(v + c), np.float64)))
+                  for c in "xyz")
+        den = max(np.max(np.abs(np.asarray(ref(v + c), np.float64))) for c in "xyz")
+        out[v.upper()] = num / den if den > 0 else num
+    return out

@@ -12,7 +12,7 @@ import math
 import numpy as np
 import pytest
 
-from helpers import FIELDS6, interior, normwise_max_rel, random_fields
+from helpers import FIELDS6, field_errors, interior, normwise_max_rel, random_fields
 from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -115,11 +115,10 @@ def test_fp64_steps_match_oracle(name):
     nsteps = min(p.steps, 12)
     print(f"case {name} n={p.n} steps={nsteps}", flush=True)
     o, g = _run_both(p, nsteps)
-    worst = 0.0
-    for f in FIELDS6:
-        worst = max(worst, normwise_max_rel(g.download(f), o.field(f)))
-    print(f"{name}: worst normwise max-rel {worst:.3e}")
-    assert worst <= 1e-12
+    err = field_errors(g.download, o.field)
+    per_comp = {f: normwise_max_rel(g.download(f), o.field(f)) for f in FIELDS6}
+    print(f"{name}: normwise max-rel E {err['E']:.3e} B {err['B']:.3e}; per component {per_comp}")
+    assert max(err.values()) <= 1e-12
     if p.bc_kind in (0, 1) and not (p.bc_kind == 1 and p.bc_params[8]):
         # no transcendental function on the path: bit for bit
         _compare_exact(o, g)
