@@ -84,9 +84,12 @@ typedef struct tff_config {
                           any TFF_ABSORB makes all three absorbing as
                           initialize.F90:143-148 does */
   int absorb_buff;     /* <grid> absorb_buff (default 20, initialize.F90:147) */
-  int absorb_fixed;    /* 0: rk3Update's absorbing layer as coded
-                          (field_solver.F90:211 uses i for jg; delta
-                          unclamped); 1: jg from j and delta clamped to 1 */
+  int absorb_fixed;    /* 0: as coded -- rk3Update's absorbing layer uses i
+                          for jg (field_solver.F90:211) and an unclamped
+                          delta, and cleanEpar divides by |B|^2 with no
+                          floor (:300); 1: fixed -- jg from j, delta clamped
+                          to 1 (fields beyond r_max are zeroed) and cleanEpar
+                          skips the projection where |B|^2 is exactly 0 */
   int precision;       /* TFF_FP64, TFF_FP32, TFF_FP32_BULK */
   int device;          /* HIP device ordinal */
   double cc;           /* <algorithm> c, the CFL number CC (global_namespace.F90:14) */

@@ -275,8 +275,13 @@ void orc_rk3_step(orc_t *o, double c1, double c2, double c3) {
   orc_rk3_update(o, (real)c1, (real)c2, (real)c3);
 }
 
-/* --- field_solver.F90:273-335  cleanEpar -------------------------------- */
+/* --- field_solver.F90:273-335  cleanEpar --------------------------------
+ * As coded the projection divides by |B|^2 with no floor, so an exact null
+ * gives NaN.  In fixed mode (absorb_fixed) the absorber zeroes the fields
+ * beyond r_max and so creates exact nulls; there the projection is skipped
+ * where |B|^2 == 0 and E is left as it is. */
 void orc_clean_epar(orc_t *o) {
+  const int guard = o->absorb_fixed;
   real *ex = o->f[F_EX], *ey = o->f[F_EY], *ez = o->f[F_EZ];
   const real *bx = o->f[F_BX], *by = o->f[F_BY], *bz = o->f[F_BZ];
   real *dex = o->f[F_DEX], *dey = o->f[F_DEY], *dez = o->f[F_DEZ];
@@ -296,7 +301,7 @@ void orc_clean_epar(orc_t *o) {
         intbz = (real)0.5 * (A(bz, 0, 0, 0) + A(bz, 0, -1, 0));
         bmag = intbx * intbx + intby * intby + intbz * intbz;
         epar = (intex * intbx + intey * intby + intez * intbz);
-        dex[c] = A(ex, 0, 0, 0) - epar * intbx / bmag;
+        dex[c] = (guard && bmag == (real)0) ? A(ex, 0, 0, 0) : A(ex, 0, 0, 0) - epar * intbx / bmag;
 
         intex = (real)0.25 * (A(ex, 0, 0, 0) + A(ex, -1, 0, 0) + A(ex, 0, 1, 0) + A(ex, -1, 1, 0));
         intey = A(ey, 0, 0, 0);
@@ -307,7 +312,7 @@ void orc_clean_epar(orc_t *o) {
         intbz = (real)0.5 * (A(bz, 0, 0, 0) + A(bz, -1, 0, 0));
         bmag = intbx * intbx + intby * intby + intbz * intbz;
         epar = (intex * intbx + intey * intby + intez * intbz);
-        dey[c] = A(ey, 0, 0, 0) - epar * intby / bmag;
+        dey[c] = (guard && bmag == (real)0) ? A(ey, 0, 0, 0) : A(ey, 0, 0, 0) - epar * intby / bmag;
 
         intex = (real)0.25 * (A(ex, 0, 0, 0) + A(ex, -1, 0, 0) + A(ex, 0, 0, 1) + A(ex, -1, 0, 1));
         intey = (real)0.25 * (A(ey, 0, 0, 0) + A(ey, 0, -1, 0) + A(ey, 0, 0, 1) + A(ey, 0, -1, 1));
@@ -318,7 +323,7 @@ void orc_clean_epar(orc_t *o) {
                                A(bz, 0, 0, 1) + A(bz, -1, 0, 1) + A(bz, -1, -1, 1) + A(bz, 0, -1, 1));
         bmag = intbx * intbx + intby * intby + intbz * intbz;
         epar = (intex * intbx + intey * intby + intez * intbz);
-        dez[c] = A(ez, 0, 0, 0) - epar * intbz / bmag;
+        dez[c] = (guard && bmag == (real)0) ? A(ez, 0, 0, 0) : A(ez, 0, 0, 0) - epar * intbz / bmag;
       }
   /* :332-334 copy back over the interior */
 #pragma omp parallel for schedule(static)

@@ -62,6 +62,9 @@ LOOP_DECK = """<time>
   h_L0          = 2.0
   L0            = {L0}
   r_L0          = 0.5
+  twist         = 1
+  beta_twist    = {beta}
+  t0            = {t0}
 """
 
 
@@ -106,15 +109,17 @@ def test_fortran_emwave_matches_oracle(tmp_path, fused):
 
 def test_fortran_loop_fused_equals_reference_order(tmp_path):
     n, last = 32, 6
+    p = problems.loop(n, L0=2.5, twist=True, twist_angle=1.0, steps=last + 1)
+    beta, t0 = p.bc_params[6], p.bc_params[7]
     outs = []
     for fused in (1, 0):
         out, _ = _run("tristan_ff_loop",
-                      LOOP_DECK.format(n=n, last=last, interval=3, fused=fused, L0=2.5),
+                      LOOP_DECK.format(n=n, last=last, interval=3, fused=fused, L0=2.5,
+                                       beta=repr(beta), t0=repr(t0)),
                       tmp_path, f"loop{fused}")
         outs.append(_read_dump(out / f"flds.{last:05d}.bin", n))
     for f in FIELDS6:
         assert np.array_equal(outs[0][f], outs[1][f]), f
-    p = problems.loop(n, L0=2.5, twist=False, steps=last + 1)
     o = Oracle(p.n, boundary=p.boundary, cc=p.cc)
     problems.load(o, p)
     o.preloop()

@@ -166,7 +166,7 @@ template <class PR>
 int launch_clean(tff_handle* h, const Vec& xe, const Vec& xb, const Vec& ye) {
   using S = typename PR::S;
   hipLaunchKernelGGL(k_clean<PR>, box_grid(h), dim3(BX, BY), 0, h->stream, h->g, cv3<S>(xe),
-                     cv3<S>(xb), mv3<S>(ye));
+                     cv3<S>(xb), mv3<S>(ye), h->cfg.absorb_fixed);
   return last_launch(h, "k_clean");
 }
 

@@ -6,7 +6,7 @@
 // ghost cells get the input's ghost value, so an output buffer carries the
 // same ghost contents the reference's in-place arrays keep.
 #pragma once
-#include "tff_common.h"
+#include "tff_cell.h"
 
 namespace tff {
 
@@ -48,116 +48,41 @@ __global__ void __launch_bounds__(BX* BY)
     yb.z[c] = xb.z[c];
     return;
   }
-  const int64_t sx = 1, sy = g.px, sz = g.plane;
-#define EX(a, b, d) ((T)xe.x[c + (a)*sx + (b)*sy + (d)*sz])
-#define EY(a, b, d) ((T)xe.y[c + (a)*sx + (b)*sy + (d)*sz])
-#define EZ(a, b, d) ((T)xe.z[c + (a)*sx + (b)*sy + (d)*sz])
-#define BXF(a, b, d) ((T)xb.x[c + (a)*sx + (b)*sy + (d)*sz])
-#define BYF(a, b, d) ((T)xb.y[c + (a)*sx + (b)*sy + (d)*sz])
-#define BZF(a, b, d) ((T)xb.z[c + (a)*sx + (b)*sy + (d)*sz])
-#define RHO(a, b, d) \
-  ((EX(a, b, d) - EX(a - 1, b, d)) + (EY(a, b, d) - EY(a, b - 1, d)) + (EZ(a, b, d) - EZ(a, b, d - 1)))
-  // curl (field_solver.F90:81-89)
+  const int64_t sy = g.px, sz = g.plane;
+  auto off = [=](int a, int b, int d) { return c + a + b * sy + d * sz; };
+  auto EX = [&](int a, int b, int d) { return (T)xe.x[off(a, b, d)]; };
+  auto EY = [&](int a, int b, int d) { return (T)xe.y[off(a, b, d)]; };
+  auto EZ = [&](int a, int b, int d) { return (T)xe.z[off(a, b, d)]; };
+  auto BXF = [&](int a, int b, int d) { return (T)xb.x[off(a, b, d)]; };
+  auto BYF = [&](int a, int b, int d) { return (T)xb.y[off(a, b, d)]; };
+  auto BZF = [&](int a, int b, int d) { return (T)xb.z[off(a, b, d)]; };
+  auto RHO = [&](int a, int b, int d) { return rho_node<T>(EX, EY, EZ, a, b, d); };
   const T tx = has_t0 ? (T)t0.x[c] : (T)0;
   const T ty = has_t0 ? (T)t0.y[c] : (T)0;
   const T tz = has_t0 ? (T)t0.z[c] : (T)0;
-  P dbx = cc * (P)((EY(0, 0, 1) - EY(0, 0, 0)) - (EZ(0, 1, 0) - EZ(0, 0, 0)));
-  P dby = cc * (P)((EZ(1, 0, 0) - EZ(0, 0, 0)) - (EX(0, 0, 1) - EX(0, 0, 0)));
-  P dbz = cc * (P)((EX(0, 1, 0) - EX(0, 0, 0)) - (EY(1, 0, 0) - EY(0, 0, 0)));
-  P dex = -cc * (P)(((BYF(0, 0, 0) - BYF(0, 0, -1)) - (BZF(0, 0, 0) - BZF(0, -1, 0))) - tx);
-  P dey = -cc * (P)(((BZF(0, 0, 0) - BZF(-1, 0, 0)) - (BXF(0, 0, 0) - BXF(0, 0, -1))) - ty);
-  P dez = -cc * (P)(((BXF(0, 0, 0) - BXF(0, -1, 0)) - (BYF(0, 0, 0) - BYF(-1, 0, 0))) - tz);
-  // charge density at the four nodes the currents need (computeRho :57-59)
-  const T r000 = RHO(0, 0, 0);
-  const T r100 = RHO(1, 0, 0);
-  const T r010 = RHO(0, 1, 0);
-  const T r001 = RHO(0, 0, 1);
-  const T h = (T)0.5, q = (T)0.25, e = (T)0.125;
-  {  // X current (:90-111)
-    T intrho = h * (r100 + r000);
-    T intex = EX(0, 0, 0);
-    T intey = q * (EY(0, 0, 0) + EY(1, 0, 0) + EY(0, -1, 0) + EY(1, -1, 0));
-    T intez = q * (EZ(0, 0, 0) + EZ(1, 0, 0) + EZ(0, 0, -1) + EZ(1, 0, -1));
-    T intbx = e * (BXF(0, 0, 0) + BXF(0, -1, 0) + BXF(1, -1, 0) + BXF(1, 0, 0) + BXF(0, 0, -1) +
-                   BXF(0, -1, -1) + BXF(1, -1, -1) + BXF(1, 0, -1));
-    T intby = h * (BYF(0, 0, 0) + BYF(0, 0, -1));
-    T intbz = h * (BZF(0, 0, 0) + BZF(0, -1, 0));
-    dex = dex - drift_current<P>(cc, (P)intrho, (P)intex, (P)intey, (P)intez, (P)intbx, (P)intby,
-                                 (P)intbz, 0);
-  }
-  {  // Y current (:113-134)
-    T intrho = h * (r010 + r000);
-    T intex = q * (EX(0, 0, 0) + EX(-1, 0, 0) + EX(0, 1, 0) + EX(-1, 1, 0));
-    T intey = EY(0, 0, 0);
-    T intez = q * (EZ(0, 0, 0) + EZ(0, 1, 0) + EZ(0, 0, -1) + EZ(0, 1, -1));
-    T intbx = h * (BXF(0, 0, 0) + BXF(0, 0, -1));
-    T intby = e * (BYF(0, 0, 0) + BYF(-1, 0, 0) + BYF(-1, 1, 0) + BYF(0, 1, 0) + BYF(0, 0, -1) +
-                   BYF(-1, 0, -1) + BYF(-1, 1, -1) + BYF(0, 1, -1));
-    T intbz = h * (BZF(0, 0, 0) + BZF(-1, 0, 0));
-    dey = dey - drift_current<P>(cc, (P)intrho, (P)intey, (P)intex, (P)intez, (P)intby, (P)intbx,
-                                 (P)intbz, 1);
-  }
-  {  // Z current (:136-157)
-    T intrho = h * (r000 + r001);
-    T intex = q * (EX(0, 0, 0) + EX(-1, 0, 0) + EX(0, 0, 1) + EX(-1, 0, 1));
-    T intey = q * (EY(0, 0, 0) + EY(0, -1, 0) + EY(0, 0, 1) + EY(0, -1, 1));
-    T intez = EZ(0, 0, 0);
-    T intbx = h * (BXF(0, 0, 0) + BXF(0, -1, 0));
-    T intby = h * (BYF(0, 0, 0) + BYF(-1, 0, 0));
-    T intbz = e * (BZF(0, 0, 0) + BZF(-1, 0, 0) + BZF(-1, -1, 0) + BZF(0, -1, 0) + BZF(0, 0, 1) +
-                   BZF(-1, 0, 1) + BZF(-1, -1, 1) + BZF(0, -1, 1));
-    dez = dez - drift_current<P>(cc, (P)intrho, (P)intez, (P)intex, (P)intey, (P)intbz, (P)intbx,
-                                 (P)intby, 2);
-  }
-  // rk3Update (:187-193): ((c1*Un + c2*U) + c3*dU), rounded once to S
-  P nex = c1 * (P)ue.x[c] + c2 * (P)xe.x[c] + c3 * dex;
-  P ney = c1 * (P)ue.y[c] + c2 * (P)xe.y[c] + c3 * dey;
-  P nez = c1 * (P)ue.z[c] + c2 * (P)xe.z[c] + c3 * dez;
-  P nbx = c1 * (P)ub.x[c] + c2 * (P)xb.x[c] + c3 * dbx;
-  P nby = c1 * (P)ub.y[c] + c2 * (P)xb.y[c] + c3 * dby;
-  P nbz = c1 * (P)ub.z[c] + c2 * (P)xb.z[c] + c3 * dbz;
-  if (ab.on) {
-    // absorbing layer (:200-265); the reference stores first and damps the
-    // stored value, so the FP32 path rounds before damping as well
-    nex = (P)(S)nex; ney = (P)(S)ney; nez = (P)(S)nez;
-    nbx = (P)(S)nbx; nby = (P)(S)nby; nbz = (P)(S)nbz;
-    const P ig = (P)i, jg = ab.fixed ? (P)j : (P)i, kg = (P)k;
-    P r = sqrt((ig - ab.xc) * (ig - ab.xc) + (jg - ab.yc) * (jg - ab.yc) + (kg - ab.zc) * (kg - ab.zc));
-    if (r > ab.rabsorb - (P)1.0) {
-      const P hh = (P)0.5, zz = (P)0.0;
-      P d;
-#define DAMP(v, ox, oy, oz)                                                                         \
-  r = sqrt((ig + (ox)-ab.xc) * (ig + (ox)-ab.xc) + (jg + (oy)-ab.yc) * (jg + (oy)-ab.yc) +         \
-           (kg + (oz)-ab.zc) * (kg + (oz)-ab.zc));                                                 \
-  d = (r - ab.rabsorb) / ab.buff;                                                                   \
-  if (ab.fixed && d > (P)1.0) d = (P)1.0;                                                           \
-  if (d > (P)0.0) v = ((P)1.0 - d * d * d) * v;
-      DAMP(nex, hh, zz, zz)
-      DAMP(ney, zz, hh, zz)
-      DAMP(nez, zz, zz, hh)
-      DAMP(nbx, zz, hh, hh)
-      DAMP(nby, hh, zz, hh)
-      DAMP(nbz, hh, hh, zz)
-#undef DAMP
-    }
-  }
-  ye.x[c] = (S)nex;
-  ye.y[c] = (S)ney;
-  ye.z[c] = (S)nez;
-  yb.x[c] = (S)nbx;
-  yb.y[c] = (S)nby;
-  yb.z[c] = (S)nbz;
-#undef RHO
+  P d[6];
+  rhs_cell<PR>(EX, EY, EZ, BXF, BYF, BZF, RHO, cc, tx, ty, tz, d);
+  P u[6] = {(P)ue.x[c], (P)ue.y[c], (P)ue.z[c], (P)ub.x[c], (P)ub.y[c], (P)ub.z[c]};
+  P v[6] = {(P)xe.x[c], (P)xe.y[c], (P)xe.z[c], (P)xb.x[c], (P)xb.y[c], (P)xb.z[c]};
+  update_cell<PR>(v, u, d, c1, c2, c3, ab, i, j, k);
+  ye.x[c] = (S)v[0];
+  ye.y[c] = (S)v[1];
+  ye.z[c] = (S)v[2];
+  yb.x[c] = (S)v[3];
+  yb.y[c] = (S)v[4];
+  yb.z[c] = (S)v[5];
 }
 
 // ---------------------------------------------------------------------------
 // cleanEpar (field_solver.F90:273-335): E <- E - (E.B) B / |B|^2 with the
 // staggered interpolants, no TINY (an exact magnetic null gives NaN exactly
-// as the reference does).  Interior only; ghosts carry the input value.
+// as the reference does).  With `guard` (the fixed mode, whose absorber
+// zeroes the fields beyond r_max) the projection is skipped where |B|^2 is
+// exactly zero.  Interior only; ghosts carry the input value.
 template <class PR>
 __global__ void __launch_bounds__(BX* BY)
     k_clean(Geom g, V3<const typename PR::S> xe, V3<const typename PR::S> xb,
-            V3<typename PR::S> ye) {
+            V3<typename PR::S> ye, int guard) {
   using S = typename PR::S;
   using T = typename PR::T;
   using P = typename PR::P;
@@ -172,58 +97,20 @@ __global__ void __launch_bounds__(BX* BY)
     ye.z[c] = xe.z[c];
     return;
   }
-  const int64_t sx = 1, sy = g.px, sz = g.plane;
-  const T h = (T)0.5, q = (T)0.25, e = (T)0.125;
-  P outx, outy, outz;
-  {
-    T intex = EX(0, 0, 0);
-    T intey = q * (EY(0, 0, 0) + EY(1, 0, 0) + EY(0, -1, 0) + EY(1, -1, 0));
-    T intez = q * (EZ(0, 0, 0) + EZ(1, 0, 0) + EZ(0, 0, -1) + EZ(1, 0, -1));
-    T intbx = e * (BXF(0, 0, 0) + BXF(0, -1, 0) + BXF(1, -1, 0) + BXF(1, 0, 0) + BXF(0, 0, -1) +
-                   BXF(0, -1, -1) + BXF(1, -1, -1) + BXF(1, 0, -1));
-    T intby = h * (BYF(0, 0, 0) + BYF(0, 0, -1));
-    T intbz = h * (BZF(0, 0, 0) + BZF(0, -1, 0));
-    P bx_ = intbx, by_ = intby, bz_ = intbz;
-    P bmag = bx_ * bx_ + by_ * by_ + bz_ * bz_;
-    P epar = ((P)intex * bx_ + (P)intey * by_ + (P)intez * bz_);
-    outx = (P)xe.x[c] - epar * bx_ / bmag;
-  }
-  {
-    T intex = q * (EX(0, 0, 0) + EX(-1, 0, 0) + EX(0, 1, 0) + EX(-1, 1, 0));
-    T intey = EY(0, 0, 0);
-    T intez = q * (EZ(0, 0, 0) + EZ(0, 1, 0) + EZ(0, 0, -1) + EZ(0, 1, -1));
-    T intbx = h * (BXF(0, 0, 0) + BXF(0, 0, -1));
-    T intby = e * (BYF(0, 0, 0) + BYF(-1, 0, 0) + BYF(-1, 1, 0) + BYF(0, 1, 0) + BYF(0, 0, -1) +
-                   BYF(-1, 0, -1) + BYF(-1, 1, -1) + BYF(0, 1, -1));
-    T intbz = h * (BZF(0, 0, 0) + BZF(-1, 0, 0));
-    P bx_ = intbx, by_ = intby, bz_ = intbz;
-    P bmag = bx_ * bx_ + by_ * by_ + bz_ * bz_;
-    P epar = ((P)intex * bx_ + (P)intey * by_ + (P)intez * bz_);
-    outy = (P)xe.y[c] - epar * by_ / bmag;
-  }
-  {
-    T intex = q * (EX(0, 0, 0) + EX(-1, 0, 0) + EX(0, 0, 1) + EX(-1, 0, 1));
-    T intey = q * (EY(0, 0, 0) + EY(0, -1, 0) + EY(0, 0, 1) + EY(0, -1, 1));
-    T intez = EZ(0, 0, 0);
-    T intbx = h * (BXF(0, 0, 0) + BXF(0, -1, 0));
-    T intby = h * (BYF(0, 0, 0) + BYF(-1, 0, 0));
-    T intbz = e * (BZF(0, 0, 0) + BZF(-1, 0, 0) + BZF(-1, -1, 0) + BZF(0, -1, 0) + BZF(0, 0, 1) +
-                   BZF(-1, 0, 1) + BZF(-1, -1, 1) + BZF(0, -1, 1));
-    P bx_ = intbx, by_ = intby, bz_ = intbz;
-    P bmag = bx_ * bx_ + by_ * by_ + bz_ * bz_;
-    P epar = ((P)intex * bx_ + (P)intey * by_ + (P)intez * bz_);
-    outz = (P)xe.z[c] - epar * bz_ / bmag;
-  }
-  ye.x[c] = (S)outx;
-  ye.y[c] = (S)outy;
-  ye.z[c] = (S)outz;
+  const int64_t sy = g.px, sz = g.plane;
+  auto off = [=](int a, int b, int d) { return c + a + b * sy + d * sz; };
+  auto EX = [&](int a, int b, int d) { return (T)xe.x[off(a, b, d)]; };
+  auto EY = [&](int a, int b, int d) { return (T)xe.y[off(a, b, d)]; };
+  auto EZ = [&](int a, int b, int d) { return (T)xe.z[off(a, b, d)]; };
+  auto BXF = [&](int a, int b, int d) { return (T)xb.x[off(a, b, d)]; };
+  auto BYF = [&](int a, int b, int d) { return (T)xb.y[off(a, b, d)]; };
+  auto BZF = [&](int a, int b, int d) { return (T)xb.z[off(a, b, d)]; };
+  P ox, oy, oz;
+  clean_cell<PR>(EX, EY, EZ, BXF, BYF, BZF, guard, ox, oy, oz);
+  ye.x[c] = (S)ox;
+  ye.y[c] = (S)oy;
+  ye.z[c] = (S)oz;
 }
-#undef EX
-#undef EY
-#undef EZ
-#undef BXF
-#undef BYF
-#undef BZF
 
 // ---------------------------------------------------------------------------
 // Ghost shell enumeration: z-slabs (all i, j), then y-slabs (interior k, all
