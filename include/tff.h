@@ -151,6 +151,19 @@ int tff_event_elapsed(tff_handle* h, int slot_a, int slot_b, float* ms);
 int tff_set_kernel_timing(tff_handle* h, int on);
 int tff_stage_kernel_time(tff_handle* h, double* total_ms, long* launches);
 
+/* Path of tff_step / tff_run.  fused = 1 (default): one fused kernel per
+ * stage applies the previous stage's cleanEpar on the fly, so the state
+ * between steps is the raw output of stage 3 and is cleaned only when the
+ * host reads it or calls a per-procedure entry point.  fused = 0: the
+ * per-procedure kernels in the order of tristanmainloop.F90:50-83.  Both
+ * give the same numbers bit for bit. */
+int tff_set_path(tff_handle* h, int fused);
+/* Tile shape (index into the built shapes; -1 = let the autotuner pick on
+ * the next step) and z-chunk of the fused stage kernel, and the current
+ * choice (tile index, its x and y extent, z-chunk). */
+int tff_set_tuning(tff_handle* h, int tile, int kchunk);
+int tff_get_tuning(tff_handle* h, int* tile, int* tx, int* ty, int* kchunk);
+
 #ifdef __cplusplus
 }
 #endif

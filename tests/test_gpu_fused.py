@@ -1,0 +1,103 @@
+"""The fused fast path (one kernel per stage, cleanEpar applied on the fly)
+against the per-procedure path and the oracle.  The fused kernel evaluates
+the same per-cell expressions, so the two paths agree bit for bit; results
+are also independent of the tile shape and z-chunk the autotuner picks."""
+import numpy as np
+import pytest
+
+from helpers import FIELDS6, field_errors
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+pytff = pytest.importorskip("pytff")
+problems = pytest.importorskip("problems")
+
+CASES = {
+    "alfven": lambda: problems.alfven(40, seed=5, kmax=6, amp=0.2),
+    "emwave_odd": lambda: problems.emwave(36, mode=(2, 1)),
+    "loop": lambda: problems.loop(40, twist_angle=1.0, steps=12),
+    "loop_shipped": lambda: problems.loop(36, twist=False, steps=12),
+    "psr": lambda: problems.psr(40, radius=5.0, r_lc=12.0, absorb_buff=6, periods=0.2),
+    "psr_faithful": lambda: problems.psr(40, radius=5.0, r_lc=12.0, absorb_buff=6, periods=0.2,
+                                         absorb_fixed=False),
+    "uniform": lambda: problems.uniform(24, L0=4.0, steps=12),
+}
+
+
+def _mb(p, precision, fused):
+    g = pytff.Meshblock(p.n, boundary=p.boundary, cc=p.cc, precision=precision,
+                        absorb_buff=p.absorb_buff, absorb_fixed=p.absorb_fixed)
+    problems.load(g, p)
+    g.preloop()
+    g.set_path(fused)
+    return g
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_bulk", "fp32"])
+@pytest.mark.parametrize("name", list(CASES))
+def test_fused_equals_unfused_bitwise(name, precision):
+    p = CASES[name]()
+    print(f"case {name} {precision} n={p.n}", flush=True)
+    a = _mb(p, precision, fused=False)
+    b = _mb(p, precision, fused=True)
+    a.run(0, 9)
+    b.run(0, 4)   # graph replay, then
+    b.step(4)     # an eager step on the raw state,
+    b.run(5, 4)   # and replay again
+    dt = np.float64 if precision == "fp64" else np.float32
+    for f in FIELDS6:
+        x, y = a.download(f, dt), b.download(f, dt)
+        assert np.array_equal(x, y, equal_nan=True), (f, float(np.nanmax(np.abs(x - y))))
+    print("tuning", b.get_tuning())
+
+
+@pytest.mark.parametrize("name", ["alfven", "loop", "psr"])
+def test_fused_fp64_matches_oracle(name):
+    p = CASES[name]()
+    o = Oracle(p.n, boundary=p.boundary, cc=p.cc, absorb_buff=p.absorb_buff,
+               absorb_fixed=p.absorb_fixed)
+    problems.load(o, p)
+    o.preloop()
+    o.run(0, 10)
+    g = _mb(p, "fp64", fused=True)
+    g.run(0, 10)
+    err = field_errors(g.download, o.field)
+    print(name, err)
+    assert max(err.values()) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["alfven", "loop"])
+def test_results_independent_of_tile_shape(name):
+    p = CASES[name]()
+    ref = None
+    for tile in range(4):
+        for kc in (8, 16, 64):
+            g = _mb(p, "fp32_bulk", fused=True)
+            try:
+                g.set_tuning(tile, kc)
+            except pytff.TffError:
+                continue  # shape too large for this device's LDS per block
+            g.run(0, 5)
+            got = {f: g.download(f, np.float32) for f in FIELDS6}
+            if ref is None:
+                ref = got
+            for f in FIELDS6:
+                assert np.array_equal(got[f], ref[f], equal_nan=True), (tile, kc, f)
+
+
+def test_reference_procedures_after_fused_steps():
+    """Per-procedure calls on a state left raw by the fused path see the
+    cleaned state (the library cleans it on demand)."""
+    p = CASES["alfven"]()
+    a = _mb(p, "fp64", fused=True)
+    b = _mb(p, "fp64", fused=False)
+    a.run(0, 3)
+    b.run(0, 3)
+    for g in (a, b):
+        g.initRKstep()
+        g.rk3Step(1.0, 0.0, 1.0)
+        g.exchangeFields()
+        g.cleanEpar()
+        g.exchangeFields()
+    for f in FIELDS6:
+        assert np.array_equal(a.download(f), b.download(f), equal_nan=True), f

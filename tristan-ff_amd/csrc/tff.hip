@@ -17,7 +17,7 @@
 #include <vector>
 
 #include "tff.h"
-#include "tff_kernels.h"
+#include "tff_fused.h"
 
 using namespace tff;
 
@@ -73,7 +73,17 @@ struct tff_handle {
   double tsum_ms = 0.0;
   long tcount = 0;
 
-  std::map<void*, hipGraphExec_t> graphs;  // keyed by the state E buffer
+  // fused path: cleaned E at ghost cells, whether the state buffers hold the
+  // cleaned (reference) state or the raw output of the last stage, and the
+  // tile shape and z-chunk picked by the autotuner
+  Vec ge;
+  bool state_clean = true;
+  int fused = 1;
+  int tile = -1;
+  int kchunk = 0;
+  int max_lds = 65536;
+
+  std::map<std::pair<void*, void*>, hipGraphExec_t> graphs;  // keyed by the state/clean E buffers
   std::vector<void*> allocs;
 };
 
@@ -292,34 +302,320 @@ int dispatch(tff_handle* h, A... a) {
   }
 }
 
+// ---- fused path -------------------------------------------------------------
+// Tile shapes the fused stage kernel is instantiated for; the autotuner
+// picks one (and the z-chunk) per handle.
+using TileA = Tile<32, 4>;
+using TileB = Tile<32, 8>;
+using TileC = Tile<64, 4>;
+using TileD = Tile<64, 8>;
+constexpr int NTILES = 4;
+constexpr int TILE_TX[NTILES] = {32, 32, 64, 64};
+constexpr int TILE_TY[NTILES] = {4, 8, 4, 8};
+
+template <class PR>
+size_t tile_lds(int t) {
+  using S = typename PR::S;
+  using T = typename PR::T;
+  switch (t) {
+    case 0: return TileA::lds_bytes<S, T>();
+    case 1: return TileB::lds_bytes<S, T>();
+    case 2: return TileC::lds_bytes<S, T>();
+    default: return TileD::lds_bytes<S, T>();
+  }
+}
+
+bool all_periodic(const tff_handle* h) {
+  return h->cfg.boundary[0] == TFF_PERIODIC && h->cfg.boundary[1] == TFF_PERIODIC &&
+         h->cfg.boundary[2] == TFF_PERIODIC;
+}
+// every ghost read is a periodic wrap and no hook writes ghosts
+bool wrap_mode(const tff_handle* h) { return all_periodic(h) && h->bc_kind == TFF_BC_NONE; }
+
+int64_t shell_count(const tff_handle* h) {
+  const int sx = h->g.nx + 2 * NG, sy = h->g.ny + 2 * NG;
+  return (int64_t)2 * NG * sx * sy + (int64_t)2 * NG * sx * h->g.nz + (int64_t)2 * NG * h->g.ny * h->g.nz;
+}
+
+template <class PR, class TL>
+int launch_stage_t(tff_handle* h, const StageArgs<typename PR::S, typename PR::P>& a) {
+  using S = typename PR::S;
+  using T = typename PR::T;
+  const size_t lds = TL::template lds_bytes<S, T>();
+  static bool attr = false;
+  if (!attr) {
+    hipFuncSetAttribute((const void*)k_stage<PR, TL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = true;
+  }
+  dim3 grid((h->g.nx + TL::TX - 1) / TL::TX, (h->g.ny + TL::TY - 1) / TL::TY,
+            (h->g.nz + a.kchunk - 1) / a.kchunk);
+  hipLaunchKernelGGL((k_stage<PR, TL>), grid, dim3(TL::NT), lds, h->stream, a);
+  return last_launch(h, "k_stage");
+}
+
+template <class PR>
+int launch_stage(tff_handle* h, const StageArgs<typename PR::S, typename PR::P>& a, int tile) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (h->timing) {
+    while (h->tpool.size() < h->tused + 2) {
+      hipEvent_t e;
+      HIPCHK(h, hipEventCreate(&e));
+      h->tpool.push_back(e);
+    }
+    e0 = h->tpool[h->tused++];
+    e1 = h->tpool[h->tused++];
+    HIPCHK(h, hipEventRecord(e0, h->stream));
+  }
+  int rc;
+  switch (tile) {
+    case 0: rc = launch_stage_t<PR, TileA>(h, a); break;
+    case 1: rc = launch_stage_t<PR, TileB>(h, a); break;
+    case 2: rc = launch_stage_t<PR, TileC>(h, a); break;
+    default: rc = launch_stage_t<PR, TileD>(h, a); break;
+  }
+  if (rc) return rc;
+  if (h->timing) HIPCHK(h, hipEventRecord(e1, h->stream));
+  return TFF_OK;
+}
+
+template <class PR>
+StageArgs<typename PR::S, typename PR::P> stage_common(tff_handle* h) {
+  using S = typename PR::S;
+  using P = typename PR::P;
+  StageArgs<S, P> a{};
+  a.g = h->g;
+  const bool exact = std::is_same<S, P>::value;
+  a.cc = exact ? (P)(S)h->cfg.cc : (P)h->cfg.cc;
+  a.ab.on = h->cfg.boundary[0] == TFF_ABSORB;
+  a.ab.fixed = h->cfg.absorb_fixed;
+  a.ab.xc = (P)0.5 * (P)h->g.nx;
+  a.ab.yc = (P)0.5 * (P)h->g.ny;
+  a.ab.zc = (P)0.5 * (P)h->g.nz;
+  a.ab.rabsorb = std::min(std::min(a.ab.xc, a.ab.yc), a.ab.zc) - (P)h->cfg.absorb_buff;
+  a.ab.buff = (P)h->cfg.absorb_buff;
+  a.t0 = cv3<S>(h->t0);
+  a.has_t0 = h->has_t0;
+  a.guard = h->cfg.absorb_fixed;
+  a.wrap = wrap_mode(h) ? 1 : 0;
+  a.kchunk = h->kchunk;
+  return a;
+}
+
+template <class PR>
+void set_coef(tff_handle* h, StageArgs<typename PR::S, typename PR::P>& a, double c1, double c2, double c3) {
+  using S = typename PR::S;
+  using P = typename PR::P;
+  const bool exact = std::is_same<S, P>::value;
+  a.c1 = exact ? (P)(S)c1 : (P)c1;
+  a.c2 = exact ? (P)(S)c2 : (P)c2;
+  a.c3 = exact ? (P)(S)c3 : (P)c3;
+}
+
+// After a fused stage wrote the interior of `oe/ob`: the reference's hook,
+// ghost fill and (for the next stage) the cleaned E of the ghost cells.
+template <class PR>
+int ghost_passes(tff_handle* h, const Vec& ge_in, const Vec& prev_b, const Vec& oe, const Vec& ob,
+                 const int* step_ptr, int step_val) {
+  using S = typename PR::S;
+  const int64_t n = shell_count(h);
+  const int tb = 256;
+  const unsigned nb = (unsigned)((n + tb - 1) / tb);
+  hipLaunchKernelGGL(k_carry<S>, dim3(nb), dim3(tb), 0, h->stream, h->g, cv3<S>(ge_in), cv3<S>(prev_b),
+                     mv3<S>(oe), mv3<S>(ob), n);
+  int rc = last_launch(h, "k_carry");
+  if (rc) return rc;
+  if ((rc = launch_bc<PR>(h, oe, ob, step_ptr, step_val))) return rc;
+  if ((rc = launch_fill<S>(h, {&oe, &ob}))) return rc;
+  const int* b = h->cfg.boundary;
+  hipLaunchKernelGGL(k_ghost_clean<PR>, dim3(nb), dim3(tb), 0, h->stream, h->g, b[0] == TFF_PERIODIC,
+                     b[1] == TFF_PERIODIC, b[2] == TFF_PERIODIC, cv3<S>(oe), cv3<S>(ob), mv3<S>(h->ge),
+                     h->cfg.absorb_fixed, n);
+  return last_launch(h, "k_ghost_clean");
+}
+
+// One timestep on the fused path.  Buffers: es/bs hold the state (raw
+// output of the last stage unless state_clean), w1/w2 the stage outputs,
+// ce the cleaned U^n E, ge the cleaned ghost E.
+template <class PR>
+int enqueue_fused_step(tff_handle* h, const int* step_ptr, int step_val) {
+  using S = typename PR::S;
+  if (!h->ge.p[0] || h->tile < 0 || h->tile >= NTILES || h->kchunk < 1)
+    return fail(h, TFF_ESTATE, "fused path not set up (ghost buffer or tuning missing)");
+  const bool wrap = wrap_mode(h);
+  const bool first = h->state_clean;
+  int rc;
+  auto a = stage_common<PR>(h);
+  // stage 1: U1 = Un + L(Un), Un = clean(state) (or the state itself when it
+  // is already clean)
+  set_coef<PR>(h, a, 1.0, 0.0, 1.0);
+  a.ye = cv3<S>(h->es);
+  a.yb = cv3<S>(h->bs);
+  a.ge = first ? cv3<S>(h->es) : cv3<S>(h->ge);
+  a.stage1 = 1;
+  a.clean_in = first ? 0 : 1;
+  a.oe = mv3<S>(h->w1e);
+  a.ob = mv3<S>(h->w1b);
+  a.u0e = first ? V3<S>{nullptr, nullptr, nullptr} : mv3<S>(h->ce);
+  if ((rc = launch_stage<PR>(h, a, h->tile))) return rc;
+  const Vec& u0e = first ? h->es : h->ce;
+  if (!wrap && (rc = ghost_passes<PR>(h, first ? h->es : h->ge, h->bs, h->w1e, h->w1b, step_ptr, step_val)))
+    return rc;
+  // stage 2: U2 = 3/4 Un + 1/4 X1 + 1/4 L(X1)
+  set_coef<PR>(h, a, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0);
+  a.ye = cv3<S>(h->w1e);
+  a.yb = cv3<S>(h->w1b);
+  a.ge = cv3<S>(h->ge);
+  a.stage1 = 0;
+  a.clean_in = 1;
+  a.ue = cv3<S>(u0e);
+  a.ub = cv3<S>(h->bs);
+  a.oe = mv3<S>(h->w2e);
+  a.ob = mv3<S>(h->w2b);
+  a.u0e = V3<S>{nullptr, nullptr, nullptr};
+  if ((rc = launch_stage<PR>(h, a, h->tile))) return rc;
+  if (!wrap && (rc = ghost_passes<PR>(h, h->ge, h->w1b, h->w2e, h->w2b, step_ptr, step_val))) return rc;
+  // stage 3: U3 = 1/3 Un + 2/3 X2 + 2/3 L(X2), written over the state in
+  // place (each thread reads Un only at its own cell before writing it)
+  set_coef<PR>(h, a, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0);
+  a.ye = cv3<S>(h->w2e);
+  a.yb = cv3<S>(h->w2b);
+  a.oe = mv3<S>(h->es);
+  a.ob = mv3<S>(h->bs);
+  if ((rc = launch_stage<PR>(h, a, h->tile))) return rc;
+  if (!wrap && (rc = ghost_passes<PR>(h, h->ge, h->w2b, h->es, h->bs, step_ptr, step_val))) return rc;
+  h->state_clean = false;
+  return TFF_OK;
+}
+
+// Turn the fused path's raw state back into the reference's arrays: the
+// cleaned interior, cleaned ghosts (periodic wrap, or the ghost-pass
+// values) and filled B ghosts.
+template <class PR>
+int ensure_clean_t(tff_handle* h) {
+  using S = typename PR::S;
+  if (h->state_clean) return TFF_OK;
+  int rc;
+  // in wrap mode the raw state's ghosts were never written: fill them first
+  // (the reference's fill before cleanEpar), then clean and fill again
+  if (wrap_mode(h) && (rc = launch_fill<S>(h, {&h->es, &h->bs}))) return rc;
+  if ((rc = launch_clean<PR>(h, h->es, h->bs, h->ce))) return rc;
+  if (wrap_mode(h)) {
+    if ((rc = launch_fill<S>(h, {&h->ce}))) return rc;
+  } else {
+    const int64_t n = shell_count(h);
+    const int tb = 256;
+    hipLaunchKernelGGL(k_carry<S>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, h->stream, h->g,
+                       cv3<S>(h->ge), cv3<S>(h->bs), mv3<S>(h->ce), V3<S>{nullptr, nullptr, nullptr}, n);
+    if ((rc = last_launch(h, "k_carry"))) return rc;
+  }
+  std::swap(h->es, h->ce);
+  h->state_clean = true;
+  invalidate_graphs(h);
+  return TFF_OK;
+}
+
+template <class PR>
+struct DoEnsureClean {
+  static int run(tff_handle* h) { return ensure_clean_t<PR>(h); }
+};
+
+// Autotuner: time every tile shape that fits the LDS limit with every
+// z-chunk on one stage-2 launch (state buffers read, work buffers written,
+// nothing the state depends on), keep the fastest.
+template <class PR>
+int autotune_t(tff_handle* h) {
+  using S = typename PR::S;
+  if (h->tile >= 0) return TFF_OK;
+  const int kcs[] = {8, 16, 32, 64};
+  float best = 1e30f;
+  int bt = -1, bk = 0;
+  hipEvent_t e0, e1;
+  HIPCHK(h, hipEventCreate(&e0));
+  HIPCHK(h, hipEventCreate(&e1));
+  const bool saved_timing = h->timing;
+  h->timing = false;
+  for (int t = 0; t < NTILES; ++t) {
+    if (tile_lds<PR>(t) > (size_t)h->max_lds) continue;
+    for (int kc : kcs) {
+      if (kc > 2 * h->g.nz && kc != 8) continue;
+      h->kchunk = kc;
+      auto a = stage_common<PR>(h);
+      set_coef<PR>(h, a, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0);
+      a.ye = cv3<S>(h->es);
+      a.yb = cv3<S>(h->bs);
+      a.ge = cv3<S>(h->es);
+      a.clean_in = 1;
+      a.ue = cv3<S>(h->es);
+      a.ub = cv3<S>(h->bs);
+      a.oe = mv3<S>(h->w2e);
+      a.ob = mv3<S>(h->w2b);
+      int rc = launch_stage<PR>(h, a, t);  // warm-up (and instruction cache)
+      if (rc) return rc;
+      HIPCHK(h, hipEventRecord(e0, h->stream));
+      for (int r = 0; r < 3; ++r)
+        if ((rc = launch_stage<PR>(h, a, t))) return rc;
+      HIPCHK(h, hipEventRecord(e1, h->stream));
+      HIPCHK(h, hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        bt = t;
+        bk = kc;
+      }
+    }
+  }
+  h->timing = saved_timing;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (bt < 0) return fail(h, TFF_EINVAL, "no tile shape fits the device's LDS");
+  h->tile = bt;
+  h->kchunk = bk;
+  return TFF_OK;
+}
+
 template <class PR>
 struct DoStep {
   static int run(tff_handle* h, int step, int use_graph) {
     int rc = ensure_t0<PR>(h);
     if (rc) return rc;
-    if (!use_graph || h->timing) {
-      rc = enqueue_step<PR>(h, nullptr, step);
-      if (rc) return rc;
-    } else {
-      void* key = h->es.p[0];
-      auto it = h->graphs.find(key);
-      if (it == h->graphs.end()) {
-        hipGraph_t graph;
-        HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_step<PR>(h, h->d_step, 0);
-        if (rc == TFF_OK) hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(64), 0, h->stream, h->d_step);
-        hipError_t e = hipStreamEndCapture(h->stream, &graph);
-        if (rc) return rc;
-        if (e != hipSuccess) return fail(h, TFF_EHIP, "hipStreamEndCapture failed");
-        hipGraphExec_t exec;
-        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
-        if (e != hipSuccess) return fail(h, TFF_EHIP, "hipGraphInstantiate failed");
-        it = h->graphs.emplace(key, exec).first;
+    if (h->fused) {
+      if ((rc = autotune_t<PR>(h))) return rc;
+      // the first step after the state was (re)made clean runs eagerly; the
+      // steady state has fixed buffer roles and is replayed from one graph
+      if (!use_graph || h->timing || h->state_clean) {
+        if ((rc = enqueue_fused_step<PR>(h, nullptr, step))) return rc;
+        if (use_graph) hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(64), 0, h->stream, h->d_step);
+        return last_launch(h, "k_bump_step");
       }
-      HIPCHK(h, hipGraphLaunch(it->second, h->stream));
+    } else {
+      if ((rc = ensure_clean_t<PR>(h))) return rc;
+      if (!use_graph || h->timing) {
+        rc = enqueue_step<PR>(h, nullptr, step);
+        if (rc) return rc;
+        std::swap(h->es, h->ce);
+        if (use_graph) hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(64), 0, h->stream, h->d_step);
+        return last_launch(h, "k_bump_step");
+      }
     }
-    std::swap(h->es, h->ce);
+    auto key = std::make_pair(h->es.p[0], h->ce.p[0]);
+    auto it = h->graphs.find(key);
+    if (it == h->graphs.end()) {
+      hipGraph_t graph;
+      HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      rc = h->fused ? enqueue_fused_step<PR>(h, h->d_step, 0) : enqueue_step<PR>(h, h->d_step, 0);
+      if (rc == TFF_OK) hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(64), 0, h->stream, h->d_step);
+      hipError_t e = hipStreamEndCapture(h->stream, &graph);
+      if (rc) return rc;
+      if (e != hipSuccess) return fail(h, TFF_EHIP, "hipStreamEndCapture failed");
+      hipGraphExec_t exec;
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (e != hipSuccess) return fail(h, TFF_EHIP, "hipGraphInstantiate failed");
+      it = h->graphs.emplace(key, exec).first;
+    }
+    HIPCHK(h, hipGraphLaunch(it->second, h->stream));
+    if (!h->fused) std::swap(h->es, h->ce);
     return TFF_OK;
   }
 };
@@ -382,6 +678,16 @@ struct DoReduce {
   }
 };
 
+template <class PR>
+struct TileLds {
+  static int run(tff_handle* h, int t, size_t* out) {
+    *out = tile_lds<PR>(t);
+    return TFF_OK;
+  }
+};
+
+int ensure_clean(tff_handle* h) { return dispatch<DoEnsureClean>(h); }
+
 Vec* field_vec(tff_handle* h, int field, int& comp) {
   comp = field % 3;
   switch (field / 3) {
@@ -426,12 +732,18 @@ int tff_create(const tff_config* cfg, tff_handle** out) {
     delete h;
     return TFF_EHIP;
   }
+  {
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, cfg->device) == hipSuccess &&
+        lds > 0)
+      h->max_lds = lds;
+  }
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return TFF_EHIP;
   }
   int rc = TFF_OK;
-  Vec* vs[] = {&h->es, &h->bs, &h->w1e, &h->w1b, &h->w2e, &h->w2b, &h->ce, &h->b0};
+  Vec* vs[] = {&h->es, &h->bs, &h->w1e, &h->w1b, &h->w2e, &h->w2b, &h->ce, &h->b0, &h->ge};
   for (Vec* v : vs)
     if ((rc = alloc_vec(h, *v))) break;
   h->red_blocks = 1024;
@@ -477,6 +789,8 @@ static int xfer(tff_handle* h, int field, void* host, int host_dtype, bool up) {
   if (field < TFF_EX || field > TFF_BNZ) return fail(h, TFF_EINVAL, "unknown field id");
   if (host_dtype != TFF_HOST_F64 && host_dtype != TFF_HOST_F32)
     return fail(h, TFF_EINVAL, "unknown host dtype");
+  int rc0 = ensure_clean(h);
+  if (rc0) return rc0;
   int comp;
   Vec* v = field_vec(h, field, comp);
   if (field >= TFF_ENX && !v->p[0]) {
@@ -531,6 +845,8 @@ int tff_set_bc(tff_handle* h, int kind, const double* params, int nparams) {
   static const int need[4] = {0, 9, 6, 6};
   if (nparams < need[kind] || nparams > 16 || (nparams > 0 && !params))
     return fail(h, TFF_EINVAL, "wrong number of BC parameters");
+  int rc0 = ensure_clean(h);
+  if (rc0) return rc0;
   std::memset(h->bc_host, 0, sizeof(h->bc_host));
   for (int t = 0; t < nparams; ++t) h->bc_host[t] = params[t];
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -565,6 +881,10 @@ int tff_set_bc(tff_handle* h, int kind, const double* params, int nparams) {
 
 int tff_init_rk_step(tff_handle* h) {
   if (!h) return TFF_EINVAL;
+  {
+    int rc0 = ensure_clean(h);
+    if (rc0) return rc0;
+  }
   for (Vec* v : {&h->en, &h->bn})
     if (!v->p[0]) {
       int rc = alloc_vec(h, *v);
@@ -588,12 +908,20 @@ int tff_init_rk_step(tff_handle* h) {
 
 int tff_rk3_step(tff_handle* h, double c1, double c2, double c3) {
   if (!h) return TFF_EINVAL;
+  {
+    int rc0 = ensure_clean(h);
+    if (rc0) return rc0;
+  }
   if (!h->en.p[0]) return fail(h, TFF_ESTATE, "tff_rk3_step before tff_init_rk_step");
   return dispatch<DoRk3>(h, c1, c2, c3);
 }
 
 int tff_clean_epar(tff_handle* h) {
   if (!h) return TFF_EINVAL;
+  {
+    int rc0 = ensure_clean(h);
+    if (rc0) return rc0;
+  }
   int rc = dispatch<DoClean>(h);
   invalidate_graphs(h);
   return rc;
@@ -601,11 +929,19 @@ int tff_clean_epar(tff_handle* h) {
 
 int tff_exchange_fields(tff_handle* h, int exchange_b0) {
   if (!h) return TFF_EINVAL;
+  {
+    int rc0 = ensure_clean(h);
+    if (rc0) return rc0;
+  }
   return dispatch<DoFill>(h, exchange_b0);
 }
 
 int tff_user_bc(tff_handle* h, int step) {
   if (!h) return TFF_EINVAL;
+  {
+    int rc0 = ensure_clean(h);
+    if (rc0) return rc0;
+  }
   return dispatch<DoBC>(h, step);
 }
 
@@ -628,6 +964,10 @@ int tff_run(tff_handle* h, int first_step, int nsteps) {
 
 int tff_reduce(tff_handle* h, double out[4]) {
   if (!h || !out) return TFF_EINVAL;
+  {
+    int rc0 = ensure_clean(h);
+    if (rc0) return rc0;
+  }
   return dispatch<DoReduce>(h, out);
 }
 
@@ -657,6 +997,39 @@ int tff_set_kernel_timing(tff_handle* h, int on) {
   h->tused = 0;
   h->tsum_ms = 0.0;
   h->tcount = 0;
+  return TFF_OK;
+}
+
+int tff_set_path(tff_handle* h, int fused) {
+  if (!h) return TFF_EINVAL;
+  int rc = ensure_clean(h);
+  if (rc) return rc;
+  h->fused = fused != 0;
+  invalidate_graphs(h);
+  return TFF_OK;
+}
+
+int tff_set_tuning(tff_handle* h, int tile, int kchunk) {
+  if (!h) return TFF_EINVAL;
+  if (tile < -1 || tile >= NTILES || (tile >= 0 && kchunk < 1)) return fail(h, TFF_EINVAL, "bad tuning");
+  if (tile >= 0) {
+    size_t lds = 0;
+    dispatch<TileLds>(h, tile, &lds);
+    if (lds > (size_t)h->max_lds) return fail(h, TFF_EINVAL, "tile shape exceeds the device's LDS per block");
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->tile = tile;
+  h->kchunk = tile >= 0 ? kchunk : 0;
+  invalidate_graphs(h);
+  return TFF_OK;
+}
+
+int tff_get_tuning(tff_handle* h, int* tile, int* tx, int* ty, int* kchunk) {
+  if (!h || !tile || !tx || !ty || !kchunk) return TFF_EINVAL;
+  *tile = h->tile;
+  *tx = h->tile >= 0 ? TILE_TX[h->tile] : 0;
+  *ty = h->tile >= 0 ? TILE_TY[h->tile] : 0;
+  *kchunk = h->kchunk;
   return TFF_OK;
 }
 

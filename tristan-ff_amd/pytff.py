@@ -85,6 +85,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.tff_set_kernel_timing.argtypes = [P, ctypes.c_int]
     lib.tff_stage_kernel_time.argtypes = [P, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_long)]
+    lib.tff_set_path.argtypes = [P, ctypes.c_int]
+    lib.tff_set_tuning.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    lib.tff_get_tuning.argtypes = [P] + [ctypes.POINTER(ctypes.c_int)] * 4
     for name in header_functions():
         if name not in ("tff_destroy", "tff_last_error"):
             getattr(lib, name).restype = ctypes.c_int
@@ -200,6 +203,19 @@ class Meshblock:
 
     def sync(self):
         self._check(self.lib.tff_sync(self.h), "tff_sync")
+
+    def set_path(self, fused: bool):
+        """fused=True: one fused kernel per stage (default); False: the
+        per-procedure kernels in the reference's order."""
+        self._check(self.lib.tff_set_path(self.h, int(bool(fused))), "tff_set_path")
+
+    def set_tuning(self, tile: int, kchunk: int = 0):
+        self._check(self.lib.tff_set_tuning(self.h, int(tile), int(kchunk)), "tff_set_tuning")
+
+    def get_tuning(self) -> dict:
+        v = [ctypes.c_int() for _ in range(4)]
+        self._check(self.lib.tff_get_tuning(self.h, *[ctypes.byref(x) for x in v]), "tff_get_tuning")
+        return {"tile": v[0].value, "tx": v[1].value, "ty": v[2].value, "kchunk": v[3].value}
 
     # ---- timing ---------------------------------------------------------
     def event(self, slot: int):
