@@ -101,17 +101,25 @@ int fail(tff_handle* h, int code, const std::string& msg) {
       return fail((h), TFF_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
   } while (0)
 
+// One allocation per field buffer: the x, y and z components follow each
+// other nalloc elements apart (Geom::cs).
 int alloc_vec(tff_handle* h, Vec& v) {
-  for (int c = 0; c < 3; ++c) {
-    void* p = nullptr;
-    if (hipMalloc(&p, (size_t)h->nalloc * h->esz) != hipSuccess)
-      return fail(h, TFF_ENOMEM, "hipMalloc of a field buffer failed");
-    if (hipMemset(p, 0, (size_t)h->nalloc * h->esz) != hipSuccess)
-      return fail(h, TFF_EHIP, "hipMemset failed");
-    h->allocs.push_back(p);
-    v.p[c] = p;
-  }
+  void* p = nullptr;
+  const size_t nb = 3 * (size_t)h->nalloc * h->esz;
+  if (hipMalloc(&p, nb) != hipSuccess) return fail(h, TFF_ENOMEM, "hipMalloc of a field buffer failed");
+  if (hipMemset(p, 0, nb) != hipSuccess) return fail(h, TFF_EHIP, "hipMemset failed");
+  h->allocs.push_back(p);
+  for (int c = 0; c < 3; ++c) v.p[c] = (char*)p + (size_t)c * h->nalloc * h->esz;
   return TFF_OK;
+}
+
+template <class S>
+const S* base_c(const Vec& v) {
+  return (const S*)v.p[0];
+}
+template <class S>
+S* base_m(const Vec& v) {
+  return (S*)v.p[0];
 }
 
 void invalidate_graphs(tff_handle* h) {
@@ -347,9 +355,12 @@ int launch_stage_t(tff_handle* h, const StageArgs<typename PR::S, typename PR::P
     hipFuncSetAttribute((const void*)k_stage<PR, TL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr = true;
   }
-  dim3 grid((h->g.nx + TL::TX - 1) / TL::TX, (h->g.ny + TL::TY - 1) / TL::TY,
-            (h->g.nz + a.kchunk - 1) / a.kchunk);
-  hipLaunchKernelGGL((k_stage<PR, TL>), grid, dim3(TL::NT), lds, h->stream, a);
+  StageArgs<typename PR::S, typename PR::P> b = a;
+  b.nbx = (h->g.nx + TL::TX - 1) / TL::TX;
+  b.nby = (h->g.ny + TL::TY - 1) / TL::TY;
+  b.nblocks = b.nbx * b.nby * ((h->g.nz + a.kchunk - 1) / a.kchunk);
+  const unsigned grid = (unsigned)((b.nblocks + 7) / 8 * 8);
+  hipLaunchKernelGGL((k_stage<PR, TL>), dim3(grid), dim3(TL::NT), lds, h->stream, b);
   return last_launch(h, "k_stage");
 }
 
@@ -393,7 +404,7 @@ StageArgs<typename PR::S, typename PR::P> stage_common(tff_handle* h) {
   a.ab.zc = (P)0.5 * (P)h->g.nz;
   a.ab.rabsorb = std::min(std::min(a.ab.xc, a.ab.yc), a.ab.zc) - (P)h->cfg.absorb_buff;
   a.ab.buff = (P)h->cfg.absorb_buff;
-  a.t0 = cv3<S>(h->t0);
+  a.t0 = base_c<S>(h->t0);
   a.has_t0 = h->has_t0;
   a.guard = h->cfg.absorb_fixed;
   a.wrap = wrap_mode(h) ? 1 : 0;
@@ -448,39 +459,39 @@ int enqueue_fused_step(tff_handle* h, const int* step_ptr, int step_val) {
   // stage 1: U1 = Un + L(Un), Un = clean(state) (or the state itself when it
   // is already clean)
   set_coef<PR>(h, a, 1.0, 0.0, 1.0);
-  a.ye = cv3<S>(h->es);
-  a.yb = cv3<S>(h->bs);
-  a.ge = first ? cv3<S>(h->es) : cv3<S>(h->ge);
+  a.ye = base_c<S>(h->es);
+  a.yb = base_c<S>(h->bs);
+  a.ge = first ? base_c<S>(h->es) : base_c<S>(h->ge);
   a.stage1 = 1;
   a.clean_in = first ? 0 : 1;
-  a.oe = mv3<S>(h->w1e);
-  a.ob = mv3<S>(h->w1b);
-  a.u0e = first ? V3<S>{nullptr, nullptr, nullptr} : mv3<S>(h->ce);
+  a.oe = base_m<S>(h->w1e);
+  a.ob = base_m<S>(h->w1b);
+  a.u0e = first ? nullptr : base_m<S>(h->ce);
   if ((rc = launch_stage<PR>(h, a, h->tile))) return rc;
   const Vec& u0e = first ? h->es : h->ce;
   if (!wrap && (rc = ghost_passes<PR>(h, first ? h->es : h->ge, h->bs, h->w1e, h->w1b, step_ptr, step_val)))
     return rc;
   // stage 2: U2 = 3/4 Un + 1/4 X1 + 1/4 L(X1)
   set_coef<PR>(h, a, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0);
-  a.ye = cv3<S>(h->w1e);
-  a.yb = cv3<S>(h->w1b);
-  a.ge = cv3<S>(h->ge);
+  a.ye = base_c<S>(h->w1e);
+  a.yb = base_c<S>(h->w1b);
+  a.ge = base_c<S>(h->ge);
   a.stage1 = 0;
   a.clean_in = 1;
-  a.ue = cv3<S>(u0e);
-  a.ub = cv3<S>(h->bs);
-  a.oe = mv3<S>(h->w2e);
-  a.ob = mv3<S>(h->w2b);
-  a.u0e = V3<S>{nullptr, nullptr, nullptr};
+  a.ue = base_c<S>(u0e);
+  a.ub = base_c<S>(h->bs);
+  a.oe = base_m<S>(h->w2e);
+  a.ob = base_m<S>(h->w2b);
+  a.u0e = nullptr;
   if ((rc = launch_stage<PR>(h, a, h->tile))) return rc;
   if (!wrap && (rc = ghost_passes<PR>(h, h->ge, h->w1b, h->w2e, h->w2b, step_ptr, step_val))) return rc;
   // stage 3: U3 = 1/3 Un + 2/3 X2 + 2/3 L(X2), written over the state in
   // place (each thread reads Un only at its own cell before writing it)
   set_coef<PR>(h, a, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0);
-  a.ye = cv3<S>(h->w2e);
-  a.yb = cv3<S>(h->w2b);
-  a.oe = mv3<S>(h->es);
-  a.ob = mv3<S>(h->bs);
+  a.ye = base_c<S>(h->w2e);
+  a.yb = base_c<S>(h->w2b);
+  a.oe = base_m<S>(h->es);
+  a.ob = base_m<S>(h->bs);
   if ((rc = launch_stage<PR>(h, a, h->tile))) return rc;
   if (!wrap && (rc = ghost_passes<PR>(h, h->ge, h->w2b, h->es, h->bs, step_ptr, step_val))) return rc;
   h->state_clean = false;
@@ -541,14 +552,14 @@ int autotune_t(tff_handle* h) {
       h->kchunk = kc;
       auto a = stage_common<PR>(h);
       set_coef<PR>(h, a, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0);
-      a.ye = cv3<S>(h->es);
-      a.yb = cv3<S>(h->bs);
-      a.ge = cv3<S>(h->es);
+      a.ye = base_c<S>(h->es);
+      a.yb = base_c<S>(h->bs);
+      a.ge = base_c<S>(h->es);
       a.clean_in = 1;
-      a.ue = cv3<S>(h->es);
-      a.ub = cv3<S>(h->bs);
-      a.oe = mv3<S>(h->w2e);
-      a.ob = mv3<S>(h->w2b);
+      a.ue = base_c<S>(h->es);
+      a.ub = base_c<S>(h->bs);
+      a.oe = base_m<S>(h->w2e);
+      a.ob = base_m<S>(h->w2b);
       int rc = launch_stage<PR>(h, a, t);  // warm-up (and instruction cache)
       if (rc) return rc;
       HIPCHK(h, hipEventRecord(e0, h->stream));
@@ -728,6 +739,8 @@ int tff_create(const tff_config* cfg, tff_handle** out) {
   h->g.plane = (int64_t)h->g.px * (cfg->ny + 2 * NG);
   h->g.org = h->xo + (int64_t)NG * h->g.px + (int64_t)NG * h->g.plane;
   h->nalloc = h->g.plane * (cfg->nz + 2 * NG) + al;
+  h->nalloc = (h->nalloc + 63) / 64 * 64;  // keep every component 256-byte aligned
+  h->g.cs = h->nalloc;
   if (hipSetDevice(cfg->device) != hipSuccess) {
     delete h;
     return TFF_EHIP;

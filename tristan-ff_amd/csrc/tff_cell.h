@@ -9,61 +9,99 @@
 
 namespace tff {
 
-// cleanEpar at one cell (field_solver.F90:288-328): returns the three
-// cleaned E components.  Interpolants: E along its own axis at the point,
-// the other two as 4-point averages, B as one 8-point and two 2-point
-// averages; epar = E.B; E <- E - epar B / |B|^2 (no TINY).  With `guard`,
-// |B|^2 == 0 leaves E unchanged.
+// cleanEpar at one cell (field_solver.F90:288-328) with sums in T2 and the
+// projection in Q2: returns the three cleaned E components.  Interpolants:
+// E along its own axis at the point, the other two as 4-point averages, B as
+// one 8-point and two 2-point averages; epar = E.B; E <- E - epar B / |B|^2
+// (no TINY).  With `guard`, |B|^2 == 0 leaves E unchanged.  `near` reports a
+// point whose |B|^2 is below NEAR_NULL times `bref` (when bref > 0).
+template <class T2, class Q2, bool FAST, class AX, class AY, class AZ, class BXa, class BYa, class BZa>
+__device__ __forceinline__ void clean_impl(const AX& EX, const AY& EY, const AZ& EZ, const BXa& BXF,
+                                           const BYa& BYF, const BZa& BZF, int guard, Q2 bref,
+                                           Q2& ox, Q2& oy, Q2& oz, bool& near) {
+  const T2 h = (T2)0.5, q = (T2)0.25, e = (T2)0.125;
+  near = false;
+  {
+    T2 intex = (T2)EX(0, 0, 0);
+    T2 intey = q * ((T2)EY(0, 0, 0) + (T2)EY(1, 0, 0) + (T2)EY(0, -1, 0) + (T2)EY(1, -1, 0));
+    T2 intez = q * ((T2)EZ(0, 0, 0) + (T2)EZ(1, 0, 0) + (T2)EZ(0, 0, -1) + (T2)EZ(1, 0, -1));
+    T2 intbx = e * ((T2)BXF(0, 0, 0) + (T2)BXF(0, -1, 0) + (T2)BXF(1, -1, 0) + (T2)BXF(1, 0, 0) +
+                    (T2)BXF(0, 0, -1) + (T2)BXF(0, -1, -1) + (T2)BXF(1, -1, -1) + (T2)BXF(1, 0, -1));
+    T2 intby = h * ((T2)BYF(0, 0, 0) + (T2)BYF(0, 0, -1));
+    T2 intbz = h * ((T2)BZF(0, 0, 0) + (T2)BZF(0, -1, 0));
+    Q2 bx_ = intbx, by_ = intby, bz_ = intbz;
+    Q2 bmag = sq3<FAST>(bx_, by_, bz_);
+    Q2 epar = FAST ? mad_<FAST>((Q2)intez, bz_, mad_<FAST>((Q2)intey, by_, (Q2)intex * bx_))
+                   : ((Q2)intex * bx_ + (Q2)intey * by_ + (Q2)intez * bz_);
+    near |= bmag < (Q2)NEAR_NULL * bref || (FAST && bmag < (Q2)1e-30f);
+    const Q2 e0 = (Q2)EX(0, 0, 0);
+    ox = (guard && bmag == (Q2)0) ? e0 : (FAST ? e0 - div_<FAST>(epar * bx_, bmag) : e0 - epar * bx_ / bmag);
+  }
+  {
+    T2 intex = q * ((T2)EX(0, 0, 0) + (T2)EX(-1, 0, 0) + (T2)EX(0, 1, 0) + (T2)EX(-1, 1, 0));
+    T2 intey = (T2)EY(0, 0, 0);
+    T2 intez = q * ((T2)EZ(0, 0, 0) + (T2)EZ(0, 1, 0) + (T2)EZ(0, 0, -1) + (T2)EZ(0, 1, -1));
+    T2 intbx = h * ((T2)BXF(0, 0, 0) + (T2)BXF(0, 0, -1));
+    T2 intby = e * ((T2)BYF(0, 0, 0) + (T2)BYF(-1, 0, 0) + (T2)BYF(-1, 1, 0) + (T2)BYF(0, 1, 0) +
+                    (T2)BYF(0, 0, -1) + (T2)BYF(-1, 0, -1) + (T2)BYF(-1, 1, -1) + (T2)BYF(0, 1, -1));
+    T2 intbz = h * ((T2)BZF(0, 0, 0) + (T2)BZF(-1, 0, 0));
+    Q2 bx_ = intbx, by_ = intby, bz_ = intbz;
+    Q2 bmag = sq3<FAST>(bx_, by_, bz_);
+    Q2 epar = FAST ? mad_<FAST>((Q2)intez, bz_, mad_<FAST>((Q2)intey, by_, (Q2)intex * bx_))
+                   : ((Q2)intex * bx_ + (Q2)intey * by_ + (Q2)intez * bz_);
+    near |= bmag < (Q2)NEAR_NULL * bref || (FAST && bmag < (Q2)1e-30f);
+    const Q2 e0 = (Q2)EY(0, 0, 0);
+    oy = (guard && bmag == (Q2)0) ? e0 : (FAST ? e0 - div_<FAST>(epar * by_, bmag) : e0 - epar * by_ / bmag);
+  }
+  {
+    T2 intex = q * ((T2)EX(0, 0, 0) + (T2)EX(-1, 0, 0) + (T2)EX(0, 0, 1) + (T2)EX(-1, 0, 1));
+    T2 intey = q * ((T2)EY(0, 0, 0) + (T2)EY(0, -1, 0) + (T2)EY(0, 0, 1) + (T2)EY(0, -1, 1));
+    T2 intez = (T2)EZ(0, 0, 0);
+    T2 intbx = h * ((T2)BXF(0, 0, 0) + (T2)BXF(0, -1, 0));
+    T2 intby = h * ((T2)BYF(0, 0, 0) + (T2)BYF(-1, 0, 0));
+    T2 intbz = e * ((T2)BZF(0, 0, 0) + (T2)BZF(-1, 0, 0) + (T2)BZF(-1, -1, 0) + (T2)BZF(0, -1, 0) +
+                    (T2)BZF(0, 0, 1) + (T2)BZF(-1, 0, 1) + (T2)BZF(-1, -1, 1) + (T2)BZF(0, -1, 1));
+    Q2 bx_ = intbx, by_ = intby, bz_ = intbz;
+    Q2 bmag = sq3<FAST>(bx_, by_, bz_);
+    Q2 epar = FAST ? mad_<FAST>((Q2)intez, bz_, mad_<FAST>((Q2)intey, by_, (Q2)intex * bx_))
+                   : ((Q2)intex * bx_ + (Q2)intey * by_ + (Q2)intez * bz_);
+    near |= bmag < (Q2)NEAR_NULL * bref || (FAST && bmag < (Q2)1e-30f);
+    const Q2 e0 = (Q2)EZ(0, 0, 0);
+    oz = (guard && bmag == (Q2)0) ? e0 : (FAST ? e0 - div_<FAST>(epar * bz_, bmag) : e0 - epar * bz_ / bmag);
+  }
+}
+
+// |B|^2 of the raw cell, the scale of the near-null test
+template <class Q2, class BXa, class BYa, class BZa>
+__device__ __forceinline__ Q2 cell_b2(const BXa& BXF, const BYa& BYF, const BZa& BZF) {
+  const Q2 a = (Q2)BXF(0, 0, 0), b = (Q2)BYF(0, 0, 0), c = (Q2)BZF(0, 0, 0);
+  return a * a + b * b + c * c;
+}
+
 template <class PR, class AX, class AY, class AZ, class BXa, class BYa, class BZa>
 __device__ __forceinline__ void clean_cell(const AX& EX, const AY& EY, const AZ& EZ, const BXa& BXF,
                                            const BYa& BYF, const BZa& BZF, int guard,
                                            typename PR::P& ox, typename PR::P& oy,
                                            typename PR::P& oz) {
   using T = typename PR::T;
+  using Q = typename PR::Q;
   using P = typename PR::P;
-  const T h = (T)0.5, q = (T)0.25, e = (T)0.125;
-  {
-    T intex = EX(0, 0, 0);
-    T intey = q * (EY(0, 0, 0) + EY(1, 0, 0) + EY(0, -1, 0) + EY(1, -1, 0));
-    T intez = q * (EZ(0, 0, 0) + EZ(1, 0, 0) + EZ(0, 0, -1) + EZ(1, 0, -1));
-    T intbx = e * (BXF(0, 0, 0) + BXF(0, -1, 0) + BXF(1, -1, 0) + BXF(1, 0, 0) + BXF(0, 0, -1) +
-                   BXF(0, -1, -1) + BXF(1, -1, -1) + BXF(1, 0, -1));
-    T intby = h * (BYF(0, 0, 0) + BYF(0, 0, -1));
-    T intbz = h * (BZF(0, 0, 0) + BZF(0, -1, 0));
-    P bx_ = intbx, by_ = intby, bz_ = intbz;
-    P bmag = bx_ * bx_ + by_ * by_ + bz_ * bz_;
-    P epar = ((P)intex * bx_ + (P)intey * by_ + (P)intez * bz_);
-    const P e0 = (P)EX(0, 0, 0);
-    ox = (guard && bmag == (P)0) ? e0 : e0 - epar * bx_ / bmag;
+  Q qx, qy, qz;
+  bool near;
+  const Q bref = PR::null_fp64 ? cell_b2<Q>(BXF, BYF, BZF) : (Q)0;
+  clean_impl<T, Q, PR::null_fp64>(EX, EY, EZ, BXF, BYF, BZF, guard, bref, qx, qy, qz, near);
+  if (PR::null_fp64 && near) {
+    double dx, dy, dz;
+    bool nn;
+    clean_impl<double, double, false>(EX, EY, EZ, BXF, BYF, BZF, guard, 0.0, dx, dy, dz, nn);
+    ox = (P)dx;
+    oy = (P)dy;
+    oz = (P)dz;
+    return;
   }
-  {
-    T intex = q * (EX(0, 0, 0) + EX(-1, 0, 0) + EX(0, 1, 0) + EX(-1, 1, 0));
-    T intey = EY(0, 0, 0);
-    T intez = q * (EZ(0, 0, 0) + EZ(0, 1, 0) + EZ(0, 0, -1) + EZ(0, 1, -1));
-    T intbx = h * (BXF(0, 0, 0) + BXF(0, 0, -1));
-    T intby = e * (BYF(0, 0, 0) + BYF(-1, 0, 0) + BYF(-1, 1, 0) + BYF(0, 1, 0) + BYF(0, 0, -1) +
-                   BYF(-1, 0, -1) + BYF(-1, 1, -1) + BYF(0, 1, -1));
-    T intbz = h * (BZF(0, 0, 0) + BZF(-1, 0, 0));
-    P bx_ = intbx, by_ = intby, bz_ = intbz;
-    P bmag = bx_ * bx_ + by_ * by_ + bz_ * bz_;
-    P epar = ((P)intex * bx_ + (P)intey * by_ + (P)intez * bz_);
-    const P e0 = (P)EY(0, 0, 0);
-    oy = (guard && bmag == (P)0) ? e0 : e0 - epar * by_ / bmag;
-  }
-  {
-    T intex = q * (EX(0, 0, 0) + EX(-1, 0, 0) + EX(0, 0, 1) + EX(-1, 0, 1));
-    T intey = q * (EY(0, 0, 0) + EY(0, -1, 0) + EY(0, 0, 1) + EY(0, -1, 1));
-    T intez = EZ(0, 0, 0);
-    T intbx = h * (BXF(0, 0, 0) + BXF(0, -1, 0));
-    T intby = h * (BYF(0, 0, 0) + BYF(-1, 0, 0));
-    T intbz = e * (BZF(0, 0, 0) + BZF(-1, 0, 0) + BZF(-1, -1, 0) + BZF(0, -1, 0) + BZF(0, 0, 1) +
-                   BZF(-1, 0, 1) + BZF(-1, -1, 1) + BZF(0, -1, 1));
-    P bx_ = intbx, by_ = intby, bz_ = intbz;
-    P bmag = bx_ * bx_ + by_ * by_ + bz_ * bz_;
-    P epar = ((P)intex * bx_ + (P)intey * by_ + (P)intez * bz_);
-    const P e0 = (P)EZ(0, 0, 0);
-    oz = (guard && bmag == (P)0) ? e0 : e0 - epar * bz_ / bmag;
-  }
+  ox = (P)qx;
+  oy = (P)qy;
+  oz = (P)qz;
 }
 
 // charge density at a node (computeRho, field_solver.F90:57-59)
@@ -71,6 +109,52 @@ template <class T, class AX, class AY, class AZ>
 __device__ __forceinline__ T rho_node(const AX& EX, const AY& EY, const AZ& EZ, int a, int b, int d) {
   return (EX(a, b, d) - EX(a - 1, b, d)) + (EY(a, b, d) - EY(a, b - 1, d)) +
          (EZ(a, b, d) - EZ(a, b, d - 1));
+}
+
+// The three drift currents of rk3Push (field_solver.F90:90-157) with sums in
+// T2 and the normalisation in Q2; j[] gets Jx, Jy, Jz.
+template <class T2, class Q2, bool FAST, class AX, class AY, class AZ, class BXa, class BYa, class BZa>
+__device__ __forceinline__ void currents_impl(const AX& EX, const AY& EY, const AZ& EZ, const BXa& BXF,
+                                              const BYa& BYF, const BZa& BZF, Q2 cc, T2 r000, T2 r100,
+                                              T2 r010, T2 r001, Q2 bref, Q2* j, bool& near) {
+  const T2 h = (T2)0.5, q = (T2)0.25, e = (T2)0.125;
+  near = false;
+  {  // X current (:90-111)
+    T2 intrho = h * (r100 + r000);
+    T2 intex = (T2)EX(0, 0, 0);
+    T2 intey = q * ((T2)EY(0, 0, 0) + (T2)EY(1, 0, 0) + (T2)EY(0, -1, 0) + (T2)EY(1, -1, 0));
+    T2 intez = q * ((T2)EZ(0, 0, 0) + (T2)EZ(1, 0, 0) + (T2)EZ(0, 0, -1) + (T2)EZ(1, 0, -1));
+    T2 intbx = e * ((T2)BXF(0, 0, 0) + (T2)BXF(0, -1, 0) + (T2)BXF(1, -1, 0) + (T2)BXF(1, 0, 0) +
+                    (T2)BXF(0, 0, -1) + (T2)BXF(0, -1, -1) + (T2)BXF(1, -1, -1) + (T2)BXF(1, 0, -1));
+    T2 intby = h * ((T2)BYF(0, 0, 0) + (T2)BYF(0, 0, -1));
+    T2 intbz = h * ((T2)BZF(0, 0, 0) + (T2)BZF(0, -1, 0));
+    j[0] = drift_current<Q2, FAST>(cc, (Q2)intrho, (Q2)intex, (Q2)intey, (Q2)intez, (Q2)intbx, (Q2)intby,
+                             (Q2)intbz, 0, bref, near);
+  }
+  {  // Y current (:113-134)
+    T2 intrho = h * (r010 + r000);
+    T2 intex = q * ((T2)EX(0, 0, 0) + (T2)EX(-1, 0, 0) + (T2)EX(0, 1, 0) + (T2)EX(-1, 1, 0));
+    T2 intey = (T2)EY(0, 0, 0);
+    T2 intez = q * ((T2)EZ(0, 0, 0) + (T2)EZ(0, 1, 0) + (T2)EZ(0, 0, -1) + (T2)EZ(0, 1, -1));
+    T2 intbx = h * ((T2)BXF(0, 0, 0) + (T2)BXF(0, 0, -1));
+    T2 intby = e * ((T2)BYF(0, 0, 0) + (T2)BYF(-1, 0, 0) + (T2)BYF(-1, 1, 0) + (T2)BYF(0, 1, 0) +
+                    (T2)BYF(0, 0, -1) + (T2)BYF(-1, 0, -1) + (T2)BYF(-1, 1, -1) + (T2)BYF(0, 1, -1));
+    T2 intbz = h * ((T2)BZF(0, 0, 0) + (T2)BZF(-1, 0, 0));
+    j[1] = drift_current<Q2, FAST>(cc, (Q2)intrho, (Q2)intey, (Q2)intex, (Q2)intez, (Q2)intby, (Q2)intbx,
+                             (Q2)intbz, 1, bref, near);
+  }
+  {  // Z current (:136-157)
+    T2 intrho = h * (r000 + r001);
+    T2 intex = q * ((T2)EX(0, 0, 0) + (T2)EX(-1, 0, 0) + (T2)EX(0, 0, 1) + (T2)EX(-1, 0, 1));
+    T2 intey = q * ((T2)EY(0, 0, 0) + (T2)EY(0, -1, 0) + (T2)EY(0, 0, 1) + (T2)EY(0, -1, 1));
+    T2 intez = (T2)EZ(0, 0, 0);
+    T2 intbx = h * ((T2)BXF(0, 0, 0) + (T2)BXF(0, -1, 0));
+    T2 intby = h * ((T2)BYF(0, 0, 0) + (T2)BYF(-1, 0, 0));
+    T2 intbz = e * ((T2)BZF(0, 0, 0) + (T2)BZF(-1, 0, 0) + (T2)BZF(-1, -1, 0) + (T2)BZF(0, -1, 0) +
+                    (T2)BZF(0, 0, 1) + (T2)BZF(-1, 0, 1) + (T2)BZF(-1, -1, 1) + (T2)BZF(0, -1, 1));
+    j[2] = drift_current<Q2, FAST>(cc, (Q2)intrho, (Q2)intez, (Q2)intex, (Q2)intey, (Q2)intbz, (Q2)intbx,
+                             (Q2)intby, 2, bref, near);
+  }
 }
 
 // rk3Push at one cell (field_solver.F90:81-162): d[0..2] = dE, d[3..5] = dB.
@@ -82,6 +166,7 @@ __device__ __forceinline__ void rhs_cell(const AX& EX, const AY& EY, const AZ& E
                                          typename PR::P cc, typename PR::T tx, typename PR::T ty,
                                          typename PR::T tz, typename PR::P* d) {
   using T = typename PR::T;
+  using Q = typename PR::Q;
   using P = typename PR::P;
   P dbx = cc * (P)((EY(0, 0, 1) - EY(0, 0, 0)) - (EZ(0, 1, 0) - EZ(0, 0, 0)));
   P dby = cc * (P)((EZ(1, 0, 0) - EZ(0, 0, 0)) - (EX(0, 0, 1) - EX(0, 0, 0)));
@@ -93,42 +178,23 @@ __device__ __forceinline__ void rhs_cell(const AX& EX, const AY& EY, const AZ& E
   const T r100 = RHO(1, 0, 0);
   const T r010 = RHO(0, 1, 0);
   const T r001 = RHO(0, 0, 1);
-  const T h = (T)0.5, q = (T)0.25, e = (T)0.125;
-  {  // X current (:90-111)
-    T intrho = h * (r100 + r000);
-    T intex = EX(0, 0, 0);
-    T intey = q * (EY(0, 0, 0) + EY(1, 0, 0) + EY(0, -1, 0) + EY(1, -1, 0));
-    T intez = q * (EZ(0, 0, 0) + EZ(1, 0, 0) + EZ(0, 0, -1) + EZ(1, 0, -1));
-    T intbx = e * (BXF(0, 0, 0) + BXF(0, -1, 0) + BXF(1, -1, 0) + BXF(1, 0, 0) + BXF(0, 0, -1) +
-                   BXF(0, -1, -1) + BXF(1, -1, -1) + BXF(1, 0, -1));
-    T intby = h * (BYF(0, 0, 0) + BYF(0, 0, -1));
-    T intbz = h * (BZF(0, 0, 0) + BZF(0, -1, 0));
-    dex = dex - drift_current<P>(cc, (P)intrho, (P)intex, (P)intey, (P)intez, (P)intbx, (P)intby,
-                                 (P)intbz, 0);
-  }
-  {  // Y current (:113-134)
-    T intrho = h * (r010 + r000);
-    T intex = q * (EX(0, 0, 0) + EX(-1, 0, 0) + EX(0, 1, 0) + EX(-1, 1, 0));
-    T intey = EY(0, 0, 0);
-    T intez = q * (EZ(0, 0, 0) + EZ(0, 1, 0) + EZ(0, 0, -1) + EZ(0, 1, -1));
-    T intbx = h * (BXF(0, 0, 0) + BXF(0, 0, -1));
-    T intby = e * (BYF(0, 0, 0) + BYF(-1, 0, 0) + BYF(-1, 1, 0) + BYF(0, 1, 0) + BYF(0, 0, -1) +
-                   BYF(-1, 0, -1) + BYF(-1, 1, -1) + BYF(0, 1, -1));
-    T intbz = h * (BZF(0, 0, 0) + BZF(-1, 0, 0));
-    dey = dey - drift_current<P>(cc, (P)intrho, (P)intey, (P)intex, (P)intez, (P)intby, (P)intbx,
-                                 (P)intbz, 1);
-  }
-  {  // Z current (:136-157)
-    T intrho = h * (r000 + r001);
-    T intex = q * (EX(0, 0, 0) + EX(-1, 0, 0) + EX(0, 0, 1) + EX(-1, 0, 1));
-    T intey = q * (EY(0, 0, 0) + EY(0, -1, 0) + EY(0, 0, 1) + EY(0, -1, 1));
-    T intez = EZ(0, 0, 0);
-    T intbx = h * (BXF(0, 0, 0) + BXF(0, -1, 0));
-    T intby = h * (BYF(0, 0, 0) + BYF(-1, 0, 0));
-    T intbz = e * (BZF(0, 0, 0) + BZF(-1, 0, 0) + BZF(-1, -1, 0) + BZF(0, -1, 0) + BZF(0, 0, 1) +
-                   BZF(-1, 0, 1) + BZF(-1, -1, 1) + BZF(0, -1, 1));
-    dez = dez - drift_current<P>(cc, (P)intrho, (P)intez, (P)intex, (P)intey, (P)intbz, (P)intbx,
-                                 (P)intby, 2);
+  Q j[3];
+  bool near;
+  const Q bref = PR::null_fp64 ? cell_b2<Q>(BXF, BYF, BZF) : (Q)0;
+  currents_impl<T, Q, PR::null_fp64>(EX, EY, EZ, BXF, BYF, BZF, (Q)cc, r000, r100, r010, r001, bref, j,
+                                     near);
+  if (PR::null_fp64 && near) {
+    double jd[3];
+    bool nn;
+    currents_impl<double, double, false>(EX, EY, EZ, BXF, BYF, BZF, (double)cc, (double)r000, (double)r100,
+                                  (double)r010, (double)r001, 0.0, jd, nn);
+    dex = dex - (P)jd[0];
+    dey = dey - (P)jd[1];
+    dez = dez - (P)jd[2];
+  } else {
+    dex = dex - (P)j[0];
+    dey = dey - (P)j[1];
+    dez = dez - (P)j[2];
   }
   d[0] = dex; d[1] = dey; d[2] = dez;
   d[3] = dbx; d[4] = dby; d[5] = dbz;
@@ -144,7 +210,9 @@ __device__ __forceinline__ void update_cell(typename PR::P* v, const typename PR
                                             const Absorb<typename PR::P>& ab, int i, int j, int k) {
   using S = typename PR::S;
   using P = typename PR::P;
-  for (int m = 0; m < 6; ++m) v[m] = c1 * u[m] + c2 * v[m] + c3 * d[m];
+  for (int m = 0; m < 6; ++m)
+    v[m] = PR::null_fp64 ? mad_<true>(c3, d[m], mad_<true>(c2, v[m], c1 * u[m]))
+                         : c1 * u[m] + c2 * v[m] + c3 * d[m];
   if (ab.on) {
     // the reference stores first and damps the stored value
     for (int m = 0; m < 6; ++m) v[m] = (P)(S)v[m];

@@ -9,6 +9,9 @@ namespace tff {
 
 constexpr int NG = 2;  // NGHOST, src/CMakeLists.txt:19
 
+// |Bbar|^2 below this fraction of the cell's own |B|^2 counts as near a null
+constexpr float NEAR_NULL = 1e-4f;
+
 // Device layout of one field component: x fastest, every x-row padded to
 // `px` elements so that the first interior cell of each row is 128-byte
 // aligned; element (i,j,k), i,j,k in [-NG, n-1+NG], lives at
@@ -18,6 +21,7 @@ struct Geom {
   int px;
   int64_t plane;
   int64_t org;
+  int64_t cs;  // elements between the x, y and z components of one field buffer
   __host__ __device__ int64_t at(int i, int j, int k) const {
     return org + i + (int64_t)j * px + (int64_t)k * plane;
   }
@@ -31,22 +35,31 @@ struct V3 {
 };
 
 // Precision policies: S = storage, T = stencil arithmetic (differences and
-// interpolation sums), P = normalisation by |B|^2, E.B projection, limiter,
-// stage accumulation and boundary geometry.
+// interpolation sums), Q = normalisation by |B|^2, E.B projection and
+// limiter, P = stage accumulation, the curl terms and boundary geometry.
+// With null_fp64, a cell whose interpolated |B|^2 is small next to its own
+// |B|^2 (a magnetic null, where the interpolation cancels) redoes the
+// interpolation, normalisation and projection in FP64.
 struct PrecF64 {
   using S = double;
   using T = double;
+  using Q = double;
   using P = double;
+  static constexpr bool null_fp64 = false;
 };
 struct PrecF32 {
   using S = float;
   using T = float;
+  using Q = float;
   using P = float;
+  static constexpr bool null_fp64 = false;
 };
 struct PrecF32Bulk {
   using S = float;
   using T = float;
+  using Q = float;
   using P = double;
+  static constexpr bool null_fp64 = true;
 };
 
 // Parameters of rk3Update's absorbing layer (field_solver.F90:198-269).
@@ -57,42 +70,73 @@ struct Absorb {
   P xc, yc, zc, rabsorb, buff;
 };
 
+// Arithmetic helpers.  FAST (the FP32-bulk policy) contracts a*b + c into one
+// fused multiply-add and divides through the hardware reciprocal (1 ulp);
+// otherwise every operation rounds separately, as the reference does.
+template <bool FAST, class Q>
+__device__ __forceinline__ Q mad_(Q a, Q b, Q c) {
+  if constexpr (FAST) return __builtin_fmaf(a, b, c);
+  return a * b + c;
+}
+template <bool FAST>
+__device__ __forceinline__ double mad_(double a, double b, double c) {
+  if constexpr (FAST) return __builtin_fma(a, b, c);
+  return a * b + c;
+}
+template <bool FAST, class Q>
+__device__ __forceinline__ Q div_(Q a, Q b) {
+  if constexpr (FAST) return a * __builtin_amdgcn_rcpf(b);
+  return a / b;
+}
+template <bool FAST>
+__device__ __forceinline__ double div_(double a, double b) {
+  return a / b;
+}
+// |a|^2 of a 3-vector in the operation order x, y, z
+template <bool FAST, class Q>
+__device__ __forceinline__ Q sq3(Q x, Q y, Q z) {
+  if constexpr (FAST) return mad_<FAST>(z, z, mad_<FAST>(y, y, x * x));
+  return x * x + y * y + z * z;
+}
+
 // Drift current component (field_solver.F90:103-111 and cyclic forms):
 // the limiter rescales the two interpolated E components transverse to
 // the current's own axis when |E| > |B|, then
 // j = CC / (|B|^2 + TINY) * (rho * (Ea*Bb - Ba*Eb)).
 // `self` is the E component along the current's axis (never rescaled).
-template <class P>
+template <class P, bool FAST = false>
 __device__ __forceinline__ P drift_current(P cc, P intrho, P self, P ea, P eb, P bs, P ba, P bb,
-                                           int comp) {
+                                           int comp, P bref, bool& near) {
   // comp 0 (x): self=Ex, ea=Ey, eb=Ez, bs=Bx, ba=By, bb=Bz
   P bmag, emag;
   if (comp == 0) {
-    bmag = bs * bs + ba * ba + bb * bb;
-    emag = self * self + ea * ea + eb * eb;
+    bmag = sq3<FAST>(bs, ba, bb);
+    emag = sq3<FAST>(self, ea, eb);
   } else if (comp == 1) {
     // ordering of the sums follows x,y,z: here self=Ey, ea=Ex, eb=Ez,
     // bs=By, ba=Bx, bb=Bz
-    bmag = ba * ba + bs * bs + bb * bb;
-    emag = ea * ea + self * self + eb * eb;
+    bmag = sq3<FAST>(ba, bs, bb);
+    emag = sq3<FAST>(ea, self, eb);
   } else {
     // self=Ez, ea=Ex, eb=Ey, bs=Bz, ba=Bx, bb=By
-    bmag = ba * ba + bb * bb + bs * bs;
-    emag = ea * ea + eb * eb + self * self;
+    bmag = sq3<FAST>(ba, bb, bs);
+    emag = sq3<FAST>(ea, eb, self);
   }
+  near |= bmag < (P)NEAR_NULL * bref;
   if (emag > bmag) {
-    P corr = sqrt(bmag / emag);
+    P corr = sqrt(div_<FAST>(bmag, emag));
     ea = ea * corr;
     eb = eb * corr;
   }
   const P tiny = (P)1e-7;  // src/defs.F90:4
   P cross;
   if (comp == 0)
-    cross = ea * bb - ba * eb;  // intey*intbz - intby*intez
+    cross = mad_<FAST>(ea, bb, -(ba * eb));  // intey*intbz - intby*intez
   else if (comp == 1)
-    cross = eb * ba - ea * bb;  // intez*intbx - intex*intbz
+    cross = mad_<FAST>(eb, ba, -(ea * bb));  // intez*intbx - intex*intbz
   else
-    cross = ea * bb - ba * eb;  // intex*intby - intbx*intey
+    cross = mad_<FAST>(ea, bb, -(ba * eb));  // intex*intby - intbx*intey
+  if constexpr (FAST) return div_<FAST>(cc * (intrho * cross), bmag + tiny);
   return cc / (bmag + tiny) * (intrho * cross);
 }
 

@@ -45,18 +45,25 @@ struct Tile {
   }
 };
 
+// Field buffers are passed as one base pointer each: component c of a
+// buffer starts g.cs elements after component c-1 (fewer kernel arguments
+// held in scalar registers).
 template <class S, class P>
 struct StageArgs {
   Geom g;
-  V3<const S> ye, yb;  // raw input fields (E, B)
-  V3<const S> ge;      // cleaned E at ghost cells (stored-ghost mode)
-  V3<const S> ue, ub;  // U^n for stages 2 and 3
-  V3<const S> t0;      // B0 curl bracket
-  V3<S> oe, ob;        // raw output fields
-  V3<S> u0e;           // stage 1: receives the cleaned E of the input
+  const S* ye;  // raw input E
+  const S* yb;  // raw input B
+  const S* ge;  // cleaned E at ghost cells (stored-ghost mode)
+  const S* ue;  // U^n E for stages 2 and 3
+  const S* ub;  // U^n B
+  const S* t0;  // B0 curl bracket
+  S* oe;        // raw output E
+  S* ob;        // raw output B
+  S* u0e;       // stage 1: receives the cleaned E of the input (or null)
   P cc, c1, c2, c3;
   Absorb<P> ab;
   int has_t0, stage1, clean_in, guard, wrap, kchunk;
+  int nbx, nby, nblocks;  // tile grid and its size
 };
 
 __device__ __forceinline__ int wrap_index(int v, int n) {
@@ -88,9 +95,19 @@ __global__ void __launch_bounds__(TL::NT) k_stage(StageArgs<typename PR::S, type
   T* rho = (T*)(cez + 3 * N1);   // [2][NN]
 
   const Geom& g = A.g;
+  const int64_t cs = g.cs;
   const int tid = threadIdx.x;
-  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
-  const int k0 = blockIdx.z * A.kchunk;
+  // Workgroups are dealt to the 8 XCDs round-robin; give each XCD one
+  // contiguous run of tiles (x fastest, then y, then z-chunk) so that the
+  // halos neighbouring tiles share are read through the same L2.
+  const int per = (A.nblocks + 7) / 8;
+  const int tile_id = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (tile_id >= A.nblocks) return;
+  const int bxi = tile_id % A.nbx;
+  const int byi = (tile_id / A.nbx) % A.nby;
+  const int bzi = tile_id / (A.nbx * A.nby);
+  const int i0 = bxi * TX, j0 = byi * TY;
+  const int k0 = bzi * A.kchunk;
   const int k1 = min(k0 + A.kchunk, g.nz);
   // tiles whose 2-cell halo stays inside the interior need no wrapping
   const bool inner_xy = i0 >= NG && i0 + TX + NG <= g.nx && j0 >= NG && j0 + TY + NG <= g.ny;
@@ -117,12 +134,12 @@ __global__ void __launch_bounds__(TL::NT) k_stage(StageArgs<typename PR::S, type
       if (c < N2) {
         const int lx = c % W2, ly = c / W2;
         const int64_t a = addr(i0 - 2 + lx, j0 - 2 + ly, p);
-        reg[m][0] = A.ye.x[a];
-        reg[m][1] = A.ye.y[a];
-        reg[m][2] = A.ye.z[a];
-        reg[m][3] = A.yb.x[a];
-        reg[m][4] = A.yb.y[a];
-        reg[m][5] = A.yb.z[a];
+        reg[m][0] = A.ye[a];
+        reg[m][1] = A.ye[a + cs];
+        reg[m][2] = A.ye[a + 2 * cs];
+        reg[m][3] = A.yb[a];
+        reg[m][4] = A.yb[a + cs];
+        reg[m][5] = A.yb[a + 2 * cs];
       }
     }
   };
@@ -167,9 +184,9 @@ __global__ void __launch_bounds__(TL::NT) k_stage(StageArgs<typename PR::S, type
         oz = rez[s3(q) * N2 + lc];
       } else if (ghost) {
         const int64_t a = g.at(min(gi, g.nx + NG - 1), min(gj, g.ny + NG - 1), q);
-        ox = A.ge.x[a];
-        oy = A.ge.y[a];
-        oz = A.ge.z[a];
+        ox = A.ge[a];
+        oy = A.ge[a + cs];
+        oz = A.ge[a + 2 * cs];
       } else {
         auto EX = [&](int a, int b, int d) { return (T)rex[s2(q + d) * N2 + lc + a + b * W2]; };
         auto EY = [&](int a, int b, int d) { return (T)rey[s2(q + d) * N2 + lc + a + b * W2]; };
@@ -242,9 +259,9 @@ __global__ void __launch_bounds__(TL::NT) k_stage(StageArgs<typename PR::S, type
       auto BZF = [&](int a, int b, int d) { return (T)rbz[s3(k + d) * N2 + lc2 + a + b * W2]; };
       auto RHO = [&](int a, int b, int d) { return rho[s2(k + d) * NN + lcn + a + b * WN]; };
       const int64_t c = g.at(gi, gj, k);
-      const T t0x = A.has_t0 ? (T)A.t0.x[c] : (T)0;
-      const T t0y = A.has_t0 ? (T)A.t0.y[c] : (T)0;
-      const T t0z = A.has_t0 ? (T)A.t0.z[c] : (T)0;
+      const T t0x = A.has_t0 ? (T)A.t0[c] : (T)0;
+      const T t0y = A.has_t0 ? (T)A.t0[c + cs] : (T)0;
+      const T t0z = A.has_t0 ? (T)A.t0[c + 2 * cs] : (T)0;
       P d[6];
       rhs_cell<PR>(EX, EY, EZ, BXF, BYF, BZF, RHO, A.cc, t0x, t0y, t0z, d);
       P v[6] = {(P)cex[s2(k) * N1 + lc1], (P)cey[s2(k) * N1 + lc1], (P)cez[s3(k) * N1 + lc1],
@@ -252,22 +269,22 @@ __global__ void __launch_bounds__(TL::NT) k_stage(StageArgs<typename PR::S, type
       P u[6];
       if (A.stage1) {
         for (int m = 0; m < 6; ++m) u[m] = v[m];
-        if (A.u0e.x) {
-          A.u0e.x[c] = cex[s2(k) * N1 + lc1];
-          A.u0e.y[c] = cey[s2(k) * N1 + lc1];
-          A.u0e.z[c] = cez[s3(k) * N1 + lc1];
+        if (A.u0e) {
+          A.u0e[c] = cex[s2(k) * N1 + lc1];
+          A.u0e[c + cs] = cey[s2(k) * N1 + lc1];
+          A.u0e[c + 2 * cs] = cez[s3(k) * N1 + lc1];
         }
       } else {
-        u[0] = (P)A.ue.x[c]; u[1] = (P)A.ue.y[c]; u[2] = (P)A.ue.z[c];
-        u[3] = (P)A.ub.x[c]; u[4] = (P)A.ub.y[c]; u[5] = (P)A.ub.z[c];
+        u[0] = (P)A.ue[c]; u[1] = (P)A.ue[c + cs]; u[2] = (P)A.ue[c + 2 * cs];
+        u[3] = (P)A.ub[c]; u[4] = (P)A.ub[c + cs]; u[5] = (P)A.ub[c + 2 * cs];
       }
       update_cell<PR>(v, u, d, A.c1, A.c2, A.c3, A.ab, gi, gj, k);
-      A.oe.x[c] = (S)v[0];
-      A.oe.y[c] = (S)v[1];
-      A.oe.z[c] = (S)v[2];
-      A.ob.x[c] = (S)v[3];
-      A.ob.y[c] = (S)v[4];
-      A.ob.z[c] = (S)v[5];
+      A.oe[c] = (S)v[0];
+      A.oe[c + cs] = (S)v[1];
+      A.oe[c + 2 * cs] = (S)v[2];
+      A.ob[c] = (S)v[3];
+      A.ob[c + cs] = (S)v[4];
+      A.ob[c + 2 * cs] = (S)v[5];
     }
     __syncthreads();
     store_bxy(k + 2);
