@@ -164,6 +164,34 @@ int tff_set_path(tff_handle* h, int fused);
 int tff_set_tuning(tff_handle* h, int tile, int kchunk);
 int tff_get_tuning(tff_handle* h, int* tile, int* tx, int* ty, int* kchunk);
 
+/* ---- batched path (parameter scans) ----------------------------------
+ * Many independent meshblocks of uneven size advanced together: per RK
+ * stage one persistent launch of the fused stage kernel works through a
+ * device-side queue of (instance, tile) items, and the ghost passes of all
+ * instances run as two batched launches; steady-state steps replay one
+ * captured HIP graph.  Each instance is an ordinary handle
+ * (tff_batch_instance) for upload, download, reductions and the pre-loop
+ * exchange + hook; supported hooks are TFF_BC_NONE and TFF_BC_LOOP on
+ * periodic or open axes. */
+typedef struct tff_batch tff_batch;
+typedef struct tff_instance {
+  int nx, ny, nz;
+  int boundary[3];
+  double cc;
+  int bc_kind;
+  int nparams;
+  double params[16];
+} tff_instance;
+int tff_batch_create(const tff_instance* inst, int n, int precision, int device, tff_batch** out);
+void tff_batch_destroy(tff_batch* b);
+const char* tff_batch_last_error(const tff_batch* b);
+/* The handle of instance i (owned by the batch; do not destroy it).  Waits
+ * for the batch's queued work. */
+tff_handle* tff_batch_instance(tff_batch* b, int i);
+/* nsteps timesteps of every instance; returns when they are done. */
+int tff_batch_run(tff_batch* b, int first_step, int nsteps);
+int tff_batch_get_tuning(tff_batch* b, int* tile, int* tx, int* ty, int* kchunk);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,7 +17,7 @@
 #include <vector>
 
 #include "tff.h"
-#include "tff_fused.h"
+#include "tff_batch.h"
 
 using namespace tff;
 
@@ -102,12 +102,17 @@ int fail(tff_handle* h, int code, const std::string& msg) {
   } while (0)
 
 // One allocation per field buffer: the x, y and z components follow each
-// other nalloc elements apart (Geom::cs).
+// other nalloc elements apart (Geom::cs).  Every transfer and fill goes on
+// the handle's stream: it is a non-blocking stream, so work on the null
+// stream (hipMemset, hipMemcpy) would not be ordered before its kernels.
 int alloc_vec(tff_handle* h, Vec& v) {
   void* p = nullptr;
   const size_t nb = 3 * (size_t)h->nalloc * h->esz;
   if (hipMalloc(&p, nb) != hipSuccess) return fail(h, TFF_ENOMEM, "hipMalloc of a field buffer failed");
-  if (hipMemset(p, 0, nb) != hipSuccess) return fail(h, TFF_EHIP, "hipMemset failed");
+  if (hipMemsetAsync(p, 0, nb, h->stream) != hipSuccess) {
+    hipFree(p);
+    return fail(h, TFF_EHIP, "hipMemsetAsync failed");
+  }
   h->allocs.push_back(p);
   for (int c = 0; c < 3; ++c) v.p[c] = (char*)p + (size_t)c * h->nalloc * h->esz;
   return TFF_OK;
@@ -766,7 +771,8 @@ int tff_create(const tff_config* cfg, tff_handle** out) {
               hipMalloc(&h->d_part, 4 * sizeof(double) * h->red_blocks) != hipSuccess ||
               hipMalloc(&h->d_out, 4 * sizeof(double)) != hipSuccess))
     rc = TFF_ENOMEM;
-  if (!rc && hipMemset(h->d_bc, 0, 16 * sizeof(double)) != hipSuccess) rc = TFF_EHIP;
+  if (!rc && hipMemsetAsync(h->d_bc, 0, 16 * sizeof(double), h->stream) != hipSuccess) rc = TFF_EHIP;
+  if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = TFF_EHIP;
   for (int s = 0; !rc && s < 16; ++s)
     if (hipEventCreate(&h->ev[s]) != hipSuccess) rc = TFF_EHIP;
   if (rc) {
@@ -829,11 +835,12 @@ static int xfer(tff_handle* h, int field, void* host, int host_dtype, bool up) {
   char* dbase = (char*)v->p[comp] + (size_t)(h->xo - NG) * h->esz;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (up)
-    HIPCHK(h, hipMemcpy2D(dbase, (size_t)h->g.px * h->esz, hp, (size_t)sx * h->esz, (size_t)sx * h->esz,
-                          (size_t)sy * sz, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy2DAsync(dbase, (size_t)h->g.px * h->esz, hp, (size_t)sx * h->esz, (size_t)sx * h->esz,
+                               (size_t)sy * sz, hipMemcpyHostToDevice, h->stream));
   else
-    HIPCHK(h, hipMemcpy2D(hp, (size_t)sx * h->esz, dbase, (size_t)h->g.px * h->esz, (size_t)sx * h->esz,
-                          (size_t)sy * sz, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy2DAsync(hp, (size_t)sx * h->esz, dbase, (size_t)h->g.px * h->esz, (size_t)sx * h->esz,
+                               (size_t)sy * sz, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));  // the host buffer is in use until here
   if (!up && hsz != h->esz) {
     if (h->esz == 4)
       for (size_t t = 0; t < n; ++t) ((double*)host)[t] = (double)((const float*)hp)[t];
@@ -863,7 +870,8 @@ int tff_set_bc(tff_handle* h, int kind, const double* params, int nparams) {
   std::memset(h->bc_host, 0, sizeof(h->bc_host));
   for (int t = 0; t < nparams; ++t) h->bc_host[t] = params[t];
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  HIPCHK(h, hipMemcpy(h->d_bc, h->bc_host, 16 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpyAsync(h->d_bc, h->bc_host, 16 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   if (h->psr_stage) {
     hipFree(h->psr_stage);
     h->psr_stage = nullptr;
@@ -966,7 +974,7 @@ int tff_step(tff_handle* h, int timestep) {
 int tff_run(tff_handle* h, int first_step, int nsteps) {
   if (!h || nsteps < 0) return TFF_EINVAL;
   if (nsteps == 0) return TFF_OK;
-  HIPCHK(h, hipMemcpyAsync(h->d_step, &first_step, sizeof(int), hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->d_step, first_step);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   for (int n = 0; n < nsteps; ++n) {
     int rc = dispatch<DoStep>(h, first_step + n, 1);
@@ -1058,6 +1066,447 @@ int tff_stage_kernel_time(tff_handle* h, double* total_ms, long* launches) {
   h->tused = 0;
   *total_ms = h->tsum_ms;
   *launches = h->tcount;
+  return TFF_OK;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Batched path: independent meshblocks advanced together, one persistent
+// queue-driven launch of the fused stage kernel per RK stage.
+
+struct tff_batch {
+  int precision = TFF_FP32_BULK;
+  int device = 0;
+  std::vector<tff_handle*> inst;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int tile = -1, kchunk = 0, grid = 0;
+  int64_t total_tiles = 0, total_shell = 0;
+  int nghost = 0;
+  bool first = true;
+  int64_t* d_tile_base = nullptr;
+  int64_t* d_shell_base = nullptr;
+  int* d_shell_inst = nullptr;
+  void* d_info = nullptr;
+  void* d_stage = nullptr;  // BatchStage[4][n]: stage 1 from a clean state, stages 1, 2, 3
+  int* d_step = nullptr;
+  hipGraphExec_t graph = nullptr;
+};
+
+namespace {
+
+int bfail(tff_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg;
+  return code;
+}
+
+#define BHIP(b, call)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return bfail((b), TFF_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+template <class PR>
+int batch_tile_lds_ok(tff_batch* b, int t) {
+  return tile_lds<PR>(t) <= (size_t)b->inst[0]->max_lds;
+}
+
+template <class PR, class TL>
+int batch_launch_stage_t(tff_batch* b, int which) {
+  using S = typename PR::S;
+  using T = typename PR::T;
+  using P = typename PR::P;
+  const size_t lds = TL::template lds_bytes<S, T>();
+  static bool attr = false;
+  if (!attr) {
+    hipFuncSetAttribute((const void*)k_batch_stage<PR, TL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = true;
+  }
+  const int n = (int)b->inst.size();
+  const BatchStage<S, P>* st = (const BatchStage<S, P>*)b->d_stage + (size_t)which * n;
+  const unsigned grid = (unsigned)std::min<int64_t>(b->grid, b->total_tiles);
+  hipLaunchKernelGGL((k_batch_stage<PR, TL>), dim3(grid), dim3(TL::NT), lds, b->stream, st, b->d_tile_base, n,
+                     b->total_tiles);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return bfail(b, TFF_EHIP, std::string("k_batch_stage: ") + hipGetErrorString(e));
+  return TFF_OK;
+}
+
+template <class PR>
+int batch_launch_stage(tff_batch* b, int which) {
+  switch (b->tile) {
+    case 0: return batch_launch_stage_t<PR, TileA>(b, which);
+    case 1: return batch_launch_stage_t<PR, TileB>(b, which);
+    case 2: return batch_launch_stage_t<PR, TileC>(b, which);
+    default: return batch_launch_stage_t<PR, TileD>(b, which);
+  }
+}
+
+template <class PR, class TL>
+int batch_grid_t(tff_batch* b) {
+  using S = typename PR::S;
+  using T = typename PR::T;
+  const size_t lds = TL::template lds_bytes<S, T>();
+  hipFuncSetAttribute((const void*)k_batch_stage<PR, TL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  int per_cu = 0;
+  BHIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_batch_stage<PR, TL>, TL::NT, lds));
+  hipDeviceProp_t prop;
+  BHIP(b, hipGetDeviceProperties(&prop, b->device));
+  b->grid = std::max(1, per_cu) * prop.multiProcessorCount;
+  return TFF_OK;
+}
+
+template <class PR>
+int batch_grid(tff_batch* b) {
+  switch (b->tile) {
+    case 0: return batch_grid_t<PR, TileA>(b);
+    case 1: return batch_grid_t<PR, TileB>(b);
+    case 2: return batch_grid_t<PR, TileC>(b);
+    default: return batch_grid_t<PR, TileD>(b);
+  }
+}
+
+// Per-instance stage descriptors and work lists for the current tile shape.
+template <class PR, class TL>
+int batch_build_t(tff_batch* b) {
+  using S = typename PR::S;
+  using P = typename PR::P;
+  const int n = (int)b->inst.size();
+  std::vector<BatchStage<S, P>> st(4 * (size_t)n);
+  std::vector<int64_t> tile_base(n), shell_base;
+  std::vector<int> shell_inst;
+  std::vector<BatchInfo> info(n);
+  int64_t tt = 0, ss = 0;
+  for (int q = 0; q < n; ++q) {
+    tff_handle* h = b->inst[q];
+    h->kchunk = b->kchunk;
+    const bool wrap = wrap_mode(h);
+    auto a = stage_common<PR>(h);
+    a.nbx = (h->g.nx + TL::TX - 1) / TL::TX;
+    a.nby = (h->g.ny + TL::TY - 1) / TL::TY;
+    a.nblocks = a.nbx * a.nby * ((h->g.nz + a.kchunk - 1) / a.kchunk);
+    tile_base[q] = tt;
+    tt += a.nblocks;
+    if (!wrap) {
+      shell_base.push_back(ss);
+      shell_inst.push_back(q);
+      ss += shell_count(h);
+    }
+    for (int w = 0; w < 4; ++w) {
+      BatchStage<S, P>& x = st[(size_t)w * n + q];
+      x.a = a;
+      x.ge_out = base_m<S>(h->ge);
+      const bool first = w == 0;
+      if (w <= 1) {
+        set_coef<PR>(h, x.a, 1.0, 0.0, 1.0);
+        x.a.ye = base_c<S>(h->es);
+        x.a.yb = base_c<S>(h->bs);
+        x.a.ge = first ? base_c<S>(h->es) : base_c<S>(h->ge);
+        x.a.stage1 = 1;
+        x.a.clean_in = first ? 0 : 1;
+        x.a.oe = base_m<S>(h->w1e);
+        x.a.ob = base_m<S>(h->w1b);
+        // stages 2 and 3 read U^n E from ce in every step, so the first step
+        // (whose input is already clean) copies it there as well
+        x.a.u0e = base_m<S>(h->ce);
+        x.ge_in = first ? base_c<S>(h->es) : base_c<S>(h->ge);
+        x.prev_b = base_c<S>(h->bs);
+      } else {
+        const Vec& u0e = h->ce;  // steady state: stage 1 wrote the cleaned U^n E there
+        x.a.stage1 = 0;
+        x.a.clean_in = 1;
+        x.a.ge = base_c<S>(h->ge);
+        x.a.ue = base_c<S>(u0e);
+        x.a.ub = base_c<S>(h->bs);
+        x.a.u0e = nullptr;
+        x.ge_in = base_c<S>(h->ge);
+        if (w == 2) {
+          set_coef<PR>(h, x.a, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0);
+          x.a.ye = base_c<S>(h->w1e);
+          x.a.yb = base_c<S>(h->w1b);
+          x.a.oe = base_m<S>(h->w2e);
+          x.a.ob = base_m<S>(h->w2b);
+          x.prev_b = base_c<S>(h->w1b);
+        } else {
+          set_coef<PR>(h, x.a, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0);
+          x.a.ye = base_c<S>(h->w2e);
+          x.a.yb = base_c<S>(h->w2b);
+          x.a.oe = base_m<S>(h->es);
+          x.a.ob = base_m<S>(h->bs);
+          x.prev_b = base_c<S>(h->w2b);
+        }
+      }
+    }
+    info[q].per[0] = h->cfg.boundary[0] == TFF_PERIODIC;
+    info[q].per[1] = h->cfg.boundary[1] == TFF_PERIODIC;
+    info[q].per[2] = h->cfg.boundary[2] == TFF_PERIODIC;
+    info[q].bc_kind = h->bc_kind;
+    info[q].bcp = h->d_bc;
+  }
+  b->total_tiles = tt;
+  b->total_shell = ss;
+  b->nghost = (int)shell_base.size();
+  // the batch stream is non-blocking: the old lists may only be freed, and
+  // the new ones filled, in its order
+  BHIP(b, hipStreamSynchronize(b->stream));
+  hipFree(b->d_stage);
+  hipFree(b->d_tile_base);
+  hipFree(b->d_shell_base);
+  hipFree(b->d_shell_inst);
+  hipFree(b->d_info);
+  b->d_stage = b->d_tile_base = nullptr;
+  b->d_shell_base = nullptr;
+  b->d_shell_inst = nullptr;
+  b->d_info = nullptr;
+  BHIP(b, hipMalloc(&b->d_stage, st.size() * sizeof(st[0])));
+  BHIP(b, hipMemcpyAsync(b->d_stage, st.data(), st.size() * sizeof(st[0]), hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipMalloc((void**)&b->d_tile_base, n * sizeof(int64_t)));
+  BHIP(b, hipMemcpyAsync(b->d_tile_base, tile_base.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipMalloc(&b->d_info, n * sizeof(BatchInfo)));
+  BHIP(b, hipMemcpyAsync(b->d_info, info.data(), n * sizeof(BatchInfo), hipMemcpyHostToDevice, b->stream));
+  if (b->nghost) {
+    BHIP(b, hipMalloc((void**)&b->d_shell_base, shell_base.size() * sizeof(int64_t)));
+    BHIP(b, hipMemcpyAsync(b->d_shell_base, shell_base.data(), shell_base.size() * sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMalloc((void**)&b->d_shell_inst, shell_inst.size() * sizeof(int)));
+    BHIP(b, hipMemcpyAsync(b->d_shell_inst, shell_inst.data(), shell_inst.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+  }
+  BHIP(b, hipStreamSynchronize(b->stream));  // the host vectors are in use until here
+  return batch_grid<PR>(b);
+}
+
+template <class PR>
+int batch_build(tff_batch* b) {
+  switch (b->tile) {
+    case 0: return batch_build_t<PR, TileA>(b);
+    case 1: return batch_build_t<PR, TileB>(b);
+    case 2: return batch_build_t<PR, TileC>(b);
+    default: return batch_build_t<PR, TileD>(b);
+  }
+}
+
+template <class PR>
+int batch_ghosts(tff_batch* b, int which) {
+  using S = typename PR::S;
+  using P = typename PR::P;
+  if (!b->nghost) return TFF_OK;
+  const int n = (int)b->inst.size();
+  const BatchStage<S, P>* st = (const BatchStage<S, P>*)b->d_stage + (size_t)which * n;
+  const int tb = 256;
+  const unsigned nb = (unsigned)((b->total_shell + tb - 1) / tb);
+  hipLaunchKernelGGL(k_batch_ghost_build<PR>, dim3(nb), dim3(tb), 0, b->stream, st, (const BatchInfo*)b->d_info,
+                     b->d_shell_base, b->d_shell_inst, b->nghost, b->total_shell, b->d_step);
+  hipLaunchKernelGGL(k_batch_ghost_clean<PR>, dim3(nb), dim3(tb), 0, b->stream, st, (const BatchInfo*)b->d_info,
+                     b->d_shell_base, b->d_shell_inst, b->nghost, b->total_shell);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return bfail(b, TFF_EHIP, std::string("batch ghost passes: ") + hipGetErrorString(e));
+  return TFF_OK;
+}
+
+// one timestep of every instance
+template <class PR>
+int batch_enqueue_step(tff_batch* b) {
+  int rc;
+  const int s1 = b->first ? 0 : 1;
+  if ((rc = batch_launch_stage<PR>(b, s1))) return rc;
+  if ((rc = batch_ghosts<PR>(b, s1))) return rc;
+  for (int w = 2; w <= 3; ++w) {
+    if ((rc = batch_launch_stage<PR>(b, w))) return rc;
+    if ((rc = batch_ghosts<PR>(b, w))) return rc;
+  }
+  hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(64), 0, b->stream, b->d_step);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return bfail(b, TFF_EHIP, hipGetErrorString(e));
+  return TFF_OK;
+}
+
+// Autotune the batch: every tile shape and z-chunk on the stage-1 launch of
+// a running step (reads the state, writes the stage-1 output and the cleaned
+// U^n E, which the clean state does not use: no state is touched).
+template <class PR>
+int batch_autotune(tff_batch* b) {
+  if (b->tile >= 0) return batch_build<PR>(b);
+  const int kcs[] = {8, 16, 32, 64};
+  float best = 1e30f;
+  int bt = -1, bk = 0;
+  hipEvent_t e0, e1;
+  BHIP(b, hipEventCreate(&e0));
+  BHIP(b, hipEventCreate(&e1));
+  for (int t = 0; t < NTILES; ++t) {
+    if (!batch_tile_lds_ok<PR>(b, t)) continue;
+    for (int kc : kcs) {
+      b->tile = t;
+      b->kchunk = kc;
+      int rc = batch_build<PR>(b);
+      if (rc) return rc;
+      if ((rc = batch_launch_stage<PR>(b, 1))) return rc;
+      BHIP(b, hipEventRecord(e0, b->stream));
+      if ((rc = batch_launch_stage<PR>(b, 1))) return rc;
+      BHIP(b, hipEventRecord(e1, b->stream));
+      BHIP(b, hipEventSynchronize(e1));
+      float ms = 0.f;
+      BHIP(b, hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        bt = t;
+        bk = kc;
+      }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (bt < 0) return bfail(b, TFF_EINVAL, "no tile shape fits the device's LDS");
+  b->tile = bt;
+  b->kchunk = bk;
+  return batch_build<PR>(b);
+}
+
+template <class PR>
+int batch_run_t(tff_batch* b, int first_step, int nsteps) {
+  int rc;
+  // every run starts from the cleaned (reference) state of each instance, so
+  // all instances share the first-step descriptors; the buffer roles this
+  // leaves are what the descriptors and the captured graph refer to
+  for (tff_handle* h : b->inst) {
+    BHIP(b, hipStreamSynchronize(h->stream));
+    if ((rc = ensure_t0<PR>(h))) return bfail(b, rc, h->err);
+    if ((rc = ensure_clean_t<PR>(h))) return bfail(b, rc, h->err);
+    BHIP(b, hipStreamSynchronize(h->stream));
+  }
+  b->first = true;
+  if (b->graph) {
+    hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  if (b->tile < 0) {
+    if ((rc = batch_autotune<PR>(b))) return rc;
+  } else if ((rc = batch_build<PR>(b))) {
+    return rc;
+  }
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, b->stream, b->d_step, first_step);
+  for (int s = 0; s < nsteps; ++s) {
+    if (b->first) {
+      if ((rc = batch_enqueue_step<PR>(b))) return rc;
+      b->first = false;
+      continue;
+    }
+    if (!b->graph) {
+      hipGraph_t g;
+      BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      rc = batch_enqueue_step<PR>(b);
+      hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) return bfail(b, TFF_EHIP, "hipStreamEndCapture failed");
+      e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) return bfail(b, TFF_EHIP, "hipGraphInstantiate failed");
+    }
+    BHIP(b, hipGraphLaunch(b->graph, b->stream));
+  }
+  BHIP(b, hipStreamSynchronize(b->stream));
+  for (tff_handle* h : b->inst) {
+    h->state_clean = false;
+    h->tile = b->tile;
+    h->kchunk = b->kchunk;
+    invalidate_graphs(h);
+  }
+  return TFF_OK;
+}
+
+template <class PR>
+struct DoBatchRun {
+  static int run(tff_batch* b, int first, int n) { return batch_run_t<PR>(b, first, n); }
+};
+
+template <template <class> class F, class... A>
+int bdispatch(tff_batch* b, A... a) {
+  switch (b->precision) {
+    case TFF_FP64: return F<PrecF64>::run(b, a...);
+    case TFF_FP32: return F<PrecF32>::run(b, a...);
+    default: return F<PrecF32Bulk>::run(b, a...);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int tff_batch_create(const tff_instance* in, int n, int precision, int device, tff_batch** out) {
+  if (!in || n < 1 || !out) return TFF_EINVAL;
+  *out = nullptr;
+  tff_batch* b = new tff_batch();
+  b->precision = precision;
+  b->device = device;
+  int rc = TFF_OK;
+  for (int q = 0; q < n && !rc; ++q) {
+    if (in[q].bc_kind != TFF_BC_NONE && in[q].bc_kind != TFF_BC_LOOP) {
+      rc = TFF_EINVAL;
+      break;
+    }
+    for (int a = 0; a < 3; ++a)
+      if (in[q].boundary[a] == TFF_ABSORB) rc = TFF_EINVAL;
+    if (rc) break;
+    tff_config c{};
+    c.nx = in[q].nx;
+    c.ny = in[q].ny;
+    c.nz = in[q].nz;
+    for (int a = 0; a < 3; ++a) c.boundary[a] = in[q].boundary[a];
+    c.absorb_buff = 0;
+    c.absorb_fixed = 0;
+    c.precision = precision;
+    c.device = device;
+    c.cc = in[q].cc;
+    tff_handle* h = nullptr;
+    if ((rc = tff_create(&c, &h))) break;
+    b->inst.push_back(h);
+    if ((rc = tff_set_bc(h, in[q].bc_kind, in[q].params, in[q].nparams))) break;
+  }
+  if (!rc && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) rc = TFF_EHIP;
+  if (!rc && hipMalloc((void**)&b->d_step, sizeof(int)) != hipSuccess) rc = TFF_ENOMEM;
+  if (rc) {
+    tff_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return TFF_OK;
+}
+
+void tff_batch_destroy(tff_batch* b) {
+  if (!b) return;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->graph) hipGraphExecDestroy(b->graph);
+  for (tff_handle* h : b->inst) tff_destroy(h);
+  hipFree(b->d_tile_base);
+  hipFree(b->d_shell_base);
+  hipFree(b->d_shell_inst);
+  hipFree(b->d_info);
+  hipFree(b->d_stage);
+  hipFree(b->d_step);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char* tff_batch_last_error(const tff_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+tff_handle* tff_batch_instance(tff_batch* b, int i) {
+  if (!b || i < 0 || i >= (int)b->inst.size()) return nullptr;
+  hipStreamSynchronize(b->stream);
+  return b->inst[i];
+}
+
+int tff_batch_run(tff_batch* b, int first_step, int nsteps) {
+  if (!b || nsteps < 0) return TFF_EINVAL;
+  if (nsteps == 0) return TFF_OK;
+  return bdispatch<DoBatchRun>(b, first_step, nsteps);
+}
+
+int tff_batch_get_tuning(tff_batch* b, int* tile, int* tx, int* ty, int* kchunk) {
+  if (!b || !tile || !tx || !ty || !kchunk) return TFF_EINVAL;
+  *tile = b->tile;
+  *tx = b->tile >= 0 ? TILE_TX[b->tile] : 0;
+  *ty = b->tile >= 0 ? TILE_TY[b->tile] : 0;
+  *kchunk = b->kchunk;
   return TFF_OK;
 }
 

@@ -1,0 +1,192 @@
+// tff_batch.h -- kernels of the batched path: many independent meshblocks
+// (a parameter scan with uneven grid sizes) advanced together.
+//
+// Per RK stage one persistent launch of the fused stage kernel works
+// through a device-side list of (instance, tile) items, so small and large
+// instances share the machine without per-instance launches.
+// Instances with stored ghosts then get their ghost cells in two batched
+// shell passes: the new ghost values (what rk3Update leaves, the floor/top
+// hook of user_loop.F90:181-244, and exchangeFields' periodic fill, applied
+// in the reference's order) and the cleaned E of every ghost cell.
+#pragma once
+#include "tff_fused.h"
+
+namespace tff {
+
+// one instance at one stage
+template <class S, class P>
+struct BatchStage {
+  StageArgs<S, P> a;  // fused stage kernel arguments (geometry, buffers)
+  const S* ge_in;     // cleaned E at the ghosts of the stage input
+  const S* prev_b;    // raw B of the stage input
+  S* ge_out;          // receives the cleaned E at the ghosts of the output
+};
+
+// per-instance constants of the ghost passes
+struct BatchInfo {
+  int per[3];         // periodic axes
+  int bc_kind;        // TFF_BC_NONE (0) or TFF_BC_LOOP (1)
+  const double* bcp;  // hook parameters (device)
+};
+
+// index of the last entry of base[0..n) that is <= t
+__device__ __forceinline__ int find_item(const int64_t* base, int n, int64_t t) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (base[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Persistent workgroups walk the (instance, tile) list with a fixed stride:
+// workgroup w takes items w, w + G, w + 2G, ... (G = grid size), so every
+// item is done exactly once with no cross-workgroup synchronisation, and
+// the loop ends for every workgroup when its next item is past the end.
+template <class PR, class TL>
+__global__ void __launch_bounds__(TL::NT)
+    k_batch_stage(const BatchStage<typename PR::S, typename PR::P>* st, const int64_t* tile_base, int ninst,
+                  int64_t total) {
+  extern __shared__ unsigned char smem_raw[];
+  for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+    __syncthreads();  // the previous item has finished with LDS
+    const int inst = find_item(tile_base, ninst, t);
+    stage_tile<PR, TL>(st[inst].a, (int)(t - tile_base[inst]), smem_raw);
+  }
+}
+
+// New ghost values of every stored-ghost instance's stage output.
+template <class PR>
+__global__ void k_batch_ghost_build(const BatchStage<typename PR::S, typename PR::P>* st,
+                                    const BatchInfo* info, const int64_t* shell_base, const int* shell_inst,
+                                    int nghost, int64_t total, const int* step_ptr) {
+  using S = typename PR::S;
+  using P = typename PR::P;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int q = find_item(shell_base, nghost, t);
+  const int inst = shell_inst[q];
+  const BatchStage<S, P>& b = st[inst];
+  const BatchInfo& in = info[inst];
+  const Geom& g = b.a.g;
+  const int64_t cs = g.cs;
+  int i, j, k;
+  shell_cell(g, t - shell_base[q], i, j, k);
+  const int64_t d = g.at(i, j, k);
+  S* oe = b.a.oe;
+  S* ob = b.a.ob;
+  const bool ox = i < 0 || i >= g.nx, oy = j < 0 || j >= g.ny, oz = k < 0 || k >= g.nz;
+  if (!((ox && !in.per[0]) || (oy && !in.per[1]) || (oz && !in.per[2]))) {
+    // exchangeFields: the wrapped interior value (the hook never writes
+    // the interior, so this is the value after the hook)
+    const int wi = i < 0 ? i + g.nx : (i >= g.nx ? i - g.nx : i);
+    const int wj = j < 0 ? j + g.ny : (j >= g.ny ? j - g.ny : j);
+    const int wk = k < 0 ? k + g.nz : (k >= g.nz ? k - g.nz : k);
+    const int64_t s = g.at(wi, wj, wk);
+    for (int c = 0; c < 3; ++c) {
+      oe[d + c * cs] = oe[s + c * cs];
+      ob[d + c * cs] = ob[s + c * cs];
+    }
+    return;
+  }
+  // carried from the stage input
+  S e3[3], b3[3];
+  for (int c = 0; c < 3; ++c) {
+    e3[c] = b.ge_in[d + c * cs];
+    b3[c] = b.prev_b[d + c * cs];
+  }
+  if (in.bc_kind == 1 && oz) {
+    // user_loop floor/top: ex, ey, bz take the k = 0 (or nz-1) value the
+    // column holds before the fill -- the new interior value, or for a ghost
+    // column the value carried from the stage input
+    const int ks = k < 0 ? 0 : g.nz - 1;
+    const int64_t s = g.at(i, j, ks);
+    const bool col_inside = !ox && !oy;
+    if (k < 0 && in.bcp[8] != 0.0) {
+      const double* p = in.bcp;
+      const int step = *step_ptr;
+      const P xc = (P)p[0], yc1 = (P)p[1], yc2 = (P)p[2], zc = (P)p[3], bnorm = (P)p[4];
+      const P r0 = (P)p[5], beta = (P)p[6], t0 = (P)p[7], cc = b.a.cc;
+      const P xg = (P)i, yg = (P)j, zg = (P)k, time = (P)step, hf = (P)0.5;
+      P rx, ry1, ry2, rz, rr1, rr2, bzl, omega, ee0;
+      rx = xg + hf - xc; ry1 = yg - yc1; ry2 = yg - yc2; rz = zg - zc;
+      rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
+      rr2 = sqrt(rx * rx + ry2 * ry2 + rz * rz);
+      bzl = bnorm * (rz / (rr1 * rr1 * rr1) - rz / (rr2 * rr2 * rr2));
+      omega = exp(-rr1 / r0) * beta * cc / r0;
+      ee0 = -omega * bzl * tanh(time / t0);
+      e3[0] = (S)(rx * ee0);
+      rx = xg - xc; ry1 = yg + hf - yc1; ry2 = yg + hf - yc2; rz = zg - zc;
+      rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
+      rr2 = sqrt(rx * rx + ry2 * ry2 + rz * rz);
+      bzl = bnorm * (rz / (rr1 * rr1 * rr1) - rz / (rr2 * rr2 * rr2));
+      omega = exp(-rr1 / r0) * beta * cc / r0;
+      ee0 = -omega * bzl * tanh(time / t0);
+      e3[1] = (S)(ry1 * ee0);
+      rx = xg + hf - xc; ry1 = yg + hf - yc1; ry2 = yg + hf - yc2; rz = zg - zc;
+      rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
+      rr2 = sqrt(rx * rx + ry2 * ry2 + rz * rz);
+      b3[2] = (S)(bnorm * (rz / (rr1 * rr1 * rr1) - rz / (rr2 * rr2 * rr2)));
+    } else if (col_inside) {
+      e3[0] = oe[s];
+      e3[1] = oe[s + cs];
+      b3[2] = ob[s + 2 * cs];
+    } else {
+      e3[0] = b.ge_in[s];
+      e3[1] = b.ge_in[s + cs];
+      b3[2] = b.prev_b[s + 2 * cs];
+    }
+  }
+  for (int c = 0; c < 3; ++c) {
+    oe[d + c * cs] = e3[c];
+    ob[d + c * cs] = b3[c];
+  }
+}
+
+// Cleaned E at every ghost cell of the stage output (k_ghost_clean for the
+// batch).
+template <class PR>
+__global__ void k_batch_ghost_clean(const BatchStage<typename PR::S, typename PR::P>* st,
+                                    const BatchInfo* info, const int64_t* shell_base, const int* shell_inst,
+                                    int nghost, int64_t total) {
+  using S = typename PR::S;
+  using T = typename PR::T;
+  using P = typename PR::P;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int q = find_item(shell_base, nghost, t);
+  const int inst = shell_inst[q];
+  const BatchStage<S, P>& b = st[inst];
+  const BatchInfo& in = info[inst];
+  const Geom& g = b.a.g;
+  const int64_t cs = g.cs;
+  int i, j, k;
+  shell_cell(g, t - shell_base[q], i, j, k);
+  const int64_t d = g.at(i, j, k);
+  const S* ye = b.a.oe;
+  const S* yb = b.a.ob;
+  const bool ox_ = i < 0 || i >= g.nx, oy_ = j < 0 || j >= g.ny, oz_ = k < 0 || k >= g.nz;
+  if ((ox_ && !in.per[0]) || (oy_ && !in.per[1]) || (oz_ && !in.per[2])) {
+    for (int c = 0; c < 3; ++c) b.ge_out[d + c * cs] = ye[d + c * cs];
+    return;
+  }
+  const int wi = i < 0 ? i + g.nx : (i >= g.nx ? i - g.nx : i);
+  const int wj = j < 0 ? j + g.ny : (j >= g.ny ? j - g.ny : j);
+  const int wk = k < 0 ? k + g.nz : (k >= g.nz ? k - g.nz : k);
+  const int64_t c0 = g.at(wi, wj, wk), sy = g.px, sz = g.plane;
+  auto off = [=](int a, int bb, int dd) { return c0 + a + bb * sy + dd * sz; };
+  auto EX = [&](int a, int bb, int dd) { return (T)ye[off(a, bb, dd)]; };
+  auto EY = [&](int a, int bb, int dd) { return (T)ye[off(a, bb, dd) + cs]; };
+  auto EZ = [&](int a, int bb, int dd) { return (T)ye[off(a, bb, dd) + 2 * cs]; };
+  auto BXF = [&](int a, int bb, int dd) { return (T)yb[off(a, bb, dd)]; };
+  auto BYF = [&](int a, int bb, int dd) { return (T)yb[off(a, bb, dd) + cs]; };
+  auto BZF = [&](int a, int bb, int dd) { return (T)yb[off(a, bb, dd) + 2 * cs]; };
+  P px, py, pz;
+  clean_cell<PR>(EX, EY, EZ, BXF, BYF, BZF, b.a.guard, px, py, pz);
+  b.ge_out[d] = (S)px;
+  b.ge_out[d + cs] = (S)py;
+  b.ge_out[d + 2 * cs] = (S)pz;
+}
+
+}  // namespace tff

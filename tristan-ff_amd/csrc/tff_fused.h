@@ -75,14 +75,15 @@ __device__ __forceinline__ int wrap_index(int v, int n) {
 __device__ __forceinline__ int s2(int p) { return p & 1; }
 __device__ __forceinline__ int s3(int p) { return (p + 6) % 3; }
 
+// One tile (tile_id in x-fastest, then y, then z-chunk order) of a stage.
 template <class PR, class TL>
-__global__ void __launch_bounds__(TL::NT) k_stage(StageArgs<typename PR::S, typename PR::P> A) {
+__device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typename PR::P>& A, int tile_id,
+                                           unsigned char* smem_raw) {
   using S = typename PR::S;
   using T = typename PR::T;
   using P = typename PR::P;
   constexpr int TX = TL::TX, TY = TL::TY, W2 = TL::W2, N2 = TL::N2, W1 = TL::W1, N1 = TL::N1;
   constexpr int WN = TL::WN, NN = TL::NN, NT = TL::NT, L2PT = TL::L2PT;
-  extern __shared__ unsigned char smem_raw[];
   S* rex = (S*)smem_raw;         // [2][N2]
   S* rey = rex + 2 * N2;         // [2][N2]
   S* rez = rey + 2 * N2;         // [3][N2]
@@ -97,12 +98,6 @@ __global__ void __launch_bounds__(TL::NT) k_stage(StageArgs<typename PR::S, type
   const Geom& g = A.g;
   const int64_t cs = g.cs;
   const int tid = threadIdx.x;
-  // Workgroups are dealt to the 8 XCDs round-robin; give each XCD one
-  // contiguous run of tiles (x fastest, then y, then z-chunk) so that the
-  // halos neighbouring tiles share are read through the same L2.
-  const int per = (A.nblocks + 7) / 8;
-  const int tile_id = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-  if (tile_id >= A.nblocks) return;
   const int bxi = tile_id % A.nbx;
   const int byi = (tile_id / A.nbx) % A.nby;
   const int bzi = tile_id / (A.nbx * A.nby);
@@ -289,6 +284,18 @@ __global__ void __launch_bounds__(TL::NT) k_stage(StageArgs<typename PR::S, type
     __syncthreads();
     store_bxy(k + 2);
   }
+}
+
+template <class PR, class TL>
+__global__ void __launch_bounds__(TL::NT) k_stage(StageArgs<typename PR::S, typename PR::P> A) {
+  extern __shared__ unsigned char smem_raw[];
+  // Workgroups are dealt to the 8 XCDs round-robin; give each XCD one
+  // contiguous run of tiles (x fastest, then y, then z-chunk) so that the
+  // halos neighbouring tiles share are read through the same L2.
+  const int per = (A.nblocks + 7) / 8;
+  const int tile_id = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (tile_id >= A.nblocks) return;
+  stage_tile<PR, TL>(A, tile_id, smem_raw);
 }
 
 // ---------------------------------------------------------------------------

@@ -544,6 +544,12 @@ __global__ void k_bump_step(int* step) {
   if (threadIdx.x == 0) *step += 1;
 }
 
+// sets a device counter in stream order (a host value copied from the stack
+// would not be)
+__global__ void k_set_int(int* p, int v) {
+  if (threadIdx.x == 0) *p = v;
+}
+
 template <class S>
 __global__ void k_copy3(int64_t n, V3<const S> a, V3<S> b) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

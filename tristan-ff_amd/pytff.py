@@ -45,6 +45,13 @@ class Config(ctypes.Structure):
                 ("device", ctypes.c_int), ("cc", ctypes.c_double)]
 
 
+class Instance(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("boundary", ctypes.c_int * 3), ("cc", ctypes.c_double),
+                ("bc_kind", ctypes.c_int), ("nparams", ctypes.c_int),
+                ("params", ctypes.c_double * 16)]
+
+
 _lib = None
 
 
@@ -85,11 +92,22 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.tff_set_kernel_timing.argtypes = [P, ctypes.c_int]
     lib.tff_stage_kernel_time.argtypes = [P, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_long)]
+    lib.tff_batch_create.argtypes = [ctypes.POINTER(Instance), ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.POINTER(P)]
+    lib.tff_batch_destroy.argtypes = [P]
+    lib.tff_batch_destroy.restype = None
+    lib.tff_batch_last_error.argtypes = [P]
+    lib.tff_batch_last_error.restype = ctypes.c_char_p
+    lib.tff_batch_instance.argtypes = [P, ctypes.c_int]
+    lib.tff_batch_instance.restype = P
+    lib.tff_batch_run.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    lib.tff_batch_get_tuning.argtypes = [P] + [ctypes.POINTER(ctypes.c_int)] * 4
     lib.tff_set_path.argtypes = [P, ctypes.c_int]
     lib.tff_set_tuning.argtypes = [P, ctypes.c_int, ctypes.c_int]
     lib.tff_get_tuning.argtypes = [P] + [ctypes.POINTER(ctypes.c_int)] * 4
     for name in header_functions():
-        if name not in ("tff_destroy", "tff_last_error"):
+        if name not in ("tff_destroy", "tff_last_error", "tff_batch_destroy",
+                        "tff_batch_last_error", "tff_batch_instance"):
             getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -126,10 +144,26 @@ class Meshblock:
         self.shape = (nz + 2 * NG, ny + 2 * NG, nx + 2 * NG)
         self.storage = np.float64 if self.precision == FP64 else np.float32
 
+    @classmethod
+    def borrow(cls, lib, h, n, boundary, cc, precision):
+        """A Meshblock view of a handle owned elsewhere (a batch instance)."""
+        self = cls.__new__(cls)
+        self.lib = lib
+        self.h = ctypes.c_void_p(h)
+        self.n = tuple(n)
+        self.cc = cc
+        self.boundary = tuple(boundary)
+        self.precision = precision
+        nx, ny, nz = self.n
+        self.shape = (nz + 2 * NG, ny + 2 * NG, nx + 2 * NG)
+        self.storage = np.float64 if precision == FP64 else np.float32
+        self._owner = False
+        return self
+
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self, "_owner", True):
             self.lib.tff_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -235,3 +269,62 @@ class Meshblock:
         self._check(self.lib.tff_stage_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(cnt)),
                     "tff_stage_kernel_time")
         return ms.value, cnt.value
+
+
+class Batch:
+    """Independent meshblocks advanced together (tff_batch_*): one persistent
+    launch of the fused stage kernel per RK stage for all of them."""
+
+    def __init__(self, specs, precision="fp32_bulk", device=0):
+        """specs: iterable of (n, boundary, cc, bc_kind, bc_params)."""
+        self.lib = load_library()
+        self.precision = _PREC_NAMES[precision] if isinstance(precision, str) else int(precision)
+        specs = list(specs)
+        arr = (Instance * len(specs))()
+        for q, (n, boundary, cc, kind, params) in enumerate(specs):
+            nx, ny, nz = (n, n, n) if np.isscalar(n) else tuple(n)
+            arr[q].nx, arr[q].ny, arr[q].nz = nx, ny, nz
+            arr[q].boundary = (ctypes.c_int * 3)(*boundary)
+            arr[q].cc = float(cc)
+            arr[q].bc_kind = int(kind)
+            arr[q].nparams = len(params)
+            arr[q].params = (ctypes.c_double * 16)(*[float(x) for x in params])
+        h = ctypes.c_void_p()
+        rc = self.lib.tff_batch_create(arr, len(specs), self.precision, int(device), ctypes.byref(h))
+        if rc != 0:
+            raise TffError(f"tff_batch_create failed with code {rc}")
+        self.h = h
+        self.members = []
+        for q in range(len(specs)):
+            ih = self.lib.tff_batch_instance(self.h, q)
+            self.members.append(Meshblock.borrow(self.lib, ih, (arr[q].nx, arr[q].ny, arr[q].nz),
+                                                 tuple(specs[q][1]), specs[q][2], self.precision))
+
+    def __len__(self):
+        return len(self.members)
+
+    def __getitem__(self, q) -> Meshblock:
+        return self.members[q]
+
+    def run(self, first_step: int, nsteps: int):
+        rc = self.lib.tff_batch_run(self.h, int(first_step), int(nsteps))
+        if rc != 0:
+            raise TffError(f"tff_batch_run failed ({rc}): {self.lib.tff_batch_last_error(self.h).decode()}")
+
+    def get_tuning(self) -> dict:
+        v = [ctypes.c_int() for _ in range(4)]
+        self.lib.tff_batch_get_tuning(self.h, *[ctypes.byref(x) for x in v])
+        return {"tile": v[0].value, "tx": v[1].value, "ty": v[2].value, "kchunk": v[3].value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            for m in self.members:
+                m.h = None
+            self.lib.tff_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
