@@ -71,8 +71,8 @@ extern "C" {
 #define TFF_HOST_F32 1
 
 /* boundary-condition kernels (userFieldBoundaryConditions of the user files) */
-#define TFF_BC_NONE 0    /* user_emwave.F90:75-78, user_dummy.F90:312-315 */
-#define TFF_BC_LOOP 1    /* user_loop.F90:181-244 */
+#define TFF_BC_NONE 0    /* user_emwave.F90:75-78, user_dummy.F90:51-54 */
+#define TFF_BC_LOOP 1    /* user_loop.F90:101-164 */
 #define TFF_BC_PSR 2     /* user_psr.F90:72-332 */
 #define TFF_BC_UNIFORM 3 /* user_uniform.F90:77-147 */
 
@@ -108,8 +108,16 @@ int tff_upload(tff_handle* h, int field, const void* host, int host_dtype);
 int tff_download(tff_handle* h, int field, void* host, int host_dtype);
 
 /* Register the problem's boundary kernel and its parameters (the values the
- * user file's userReadInput computes; the order is listed per kind in
- * INTEGRATION.md). */
+ * user file's userReadInput computes), in this order:
+ *  TFF_BC_NONE     none
+ *  TFF_BC_LOOP     xc, yc1, yc2, zc, bnorm, r0, beta_twist, t0, twist
+ *                  (user_loop.F90:27-42 and :101-164; twist != 0 turns on
+ *                  the footpoint driver of :122-150)
+ *  TFF_BC_PSR      psr_radius, psr_angle [rad], psr_omega, xc, yc, zc
+ *                  (user_psr.F90:26-37 and :72-332)
+ *  TFF_BC_UNIFORM  xc, yc, r0, beta_twist, t0, bnorm
+ *                  (user_uniform.F90:27-69 and :77-147)
+ * Positions are in cells of this meshblock (global = local here). */
 int tff_set_bc(tff_handle* h, int kind, const double* params, int nparams);
 
 /* field_solver.F90:15-33 initRKstep: En,Bn <- E,B. */
@@ -163,11 +171,17 @@ int tff_set_path(tff_handle* h, int fused);
  * choice (tile index, its x and y extent, z-chunk). */
 int tff_set_tuning(tff_handle* h, int tile, int kchunk);
 int tff_get_tuning(tff_handle* h, int* tile, int* tx, int* ty, int* kchunk);
+/* Device layout of one field component (no reference counterpart: the
+ * reference's arrays are host Fortran arrays): out[0] = padded x-row pitch,
+ * out[1] = plane pitch, out[2] = offset of interior cell (0,0,0), out[3] =
+ * elements per component (nalloc), all in elements; out[4] = bytes per
+ * element. */
+int tff_get_layout(tff_handle* h, long long out[5]);
 
 /* ---- batched path (parameter scans) ----------------------------------
  * Many independent meshblocks of uneven size advanced together: per RK
  * stage one persistent launch of the fused stage kernel works through a
- * device-side queue of (instance, tile) items, and the ghost passes of all
+ * device-side list of (instance, tile) items, and the ghost passes of all
  * instances run as two batched launches; steady-state steps replay one
  * captured HIP graph.  Each instance is an ordinary handle
  * (tff_batch_instance) for upload, download, reductions and the pre-loop

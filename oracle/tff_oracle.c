@@ -377,10 +377,10 @@ void orc_exchange_fields(orc_t *o, int exchange_b0) {
       }
 }
 
-/* --- user/user_loop.F90:181-244  userFieldBoundaryConditions ------------
+/* --- user/user_loop.F90:101-164  userFieldBoundaryConditions ------------
  * bcp: 0 xc, 1 yc1, 2 yc2, 3 zc, 4 bnorm, 5 r0, 6 beta_twist, 7 t0,
  *      8 twist (0: as shipped, floor copy only; 1: the commented-out
- *        footpoint-rotation driver of user_loop.F90:202-230, with the
+ *        footpoint-rotation driver of user_loop.F90:122-150, with the
  *        tanh(time/t0) ramp switched on). */
 static void orc_bc_loop(orc_t *o, int step) {
   real *ex = o->f[F_EX], *ey = o->f[F_EY], *bz = o->f[F_BZ];

@@ -1054,6 +1054,16 @@ int tff_get_tuning(tff_handle* h, int* tile, int* tx, int* ty, int* kchunk) {
   return TFF_OK;
 }
 
+int tff_get_layout(tff_handle* h, long long out[5]) {
+  if (!h || !out) return TFF_EINVAL;
+  out[0] = h->g.px;
+  out[1] = h->g.plane;
+  out[2] = h->g.org;
+  out[3] = h->nalloc;
+  out[4] = (long long)h->esz;
+  return TFF_OK;
+}
+
 int tff_stage_kernel_time(tff_handle* h, double* total_ms, long* launches) {
   if (!h || !total_ms || !launches) return TFF_EINVAL;
   HIPCHK(h, hipStreamSynchronize(h->stream));

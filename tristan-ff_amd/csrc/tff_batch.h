@@ -6,7 +6,7 @@
 // instances share the machine without per-instance launches.
 // Instances with stored ghosts then get their ghost cells in two batched
 // shell passes: the new ghost values (what rk3Update leaves, the floor/top
-// hook of user_loop.F90:181-244, and exchangeFields' periodic fill, applied
+// hook of user_loop.F90:101-164, and exchangeFields' periodic fill, applied
 // in the reference's order) and the cleaned E of every ghost cell.
 #pragma once
 #include "tff_fused.h"

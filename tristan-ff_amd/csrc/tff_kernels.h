@@ -196,9 +196,9 @@ __global__ void __launch_bounds__(BX* BY)
 // ---------------------------------------------------------------------------
 // Boundary hooks.
 
-// user_loop.F90:181-244.  p: 0 xc, 1 yc1, 2 yc2, 3 zc, 4 bnorm, 5 r0,
+// user_loop.F90:101-164.  p: 0 xc, 1 yc1, 2 yc2, 3 zc, 4 bnorm, 5 r0,
 // 6 beta_twist, 7 t0, 8 twist flag (0: floor/top copy as shipped; 1: the
-// footpoint-rotation driver of the commented block :202-230 with its
+// footpoint-rotation driver of the commented block :122-150 with its
 // tanh(time/t0) ramp on).  Only z-ghost planes are written.
 template <class PR>
 __global__ void k_bc_loop(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, const double* p,

@@ -153,12 +153,12 @@ def alfven(n=256, seed=1, amp=0.05, kmax=8, cc=0.45, steps=500) -> Problem:
 # --- config: twisted loop (user_loop.F90 form) -----------------------------
 def loop(n=384, L0=None, a_L0=3.0, h_L0=2.0, r_L0=0.5, cc=0.45, twist_angle=None, seed=3,
          t0=None, steps=1000, twist=True, dtype=np.float64) -> Problem:
-    """user_loop.F90:107-173: two opposite sources at depth h*L0 below the
+    """user_loop.F90:27-93: two opposite sources at depth h*L0 below the
     floor, +-a*L0 either side of the box centre; B0 = B everywhere including
     ghosts; x, y periodic, z open with the floor/top hook.  L0 scales as
     20*n/256 (inputs/input.loop:34 on 256 cells).  With twist=True the
     floor hook also rotates footpoint 1 (the driver of the commented block
-    user_loop.F90:202-230, r0 = r_L0*L0 as user_uniform.F90:38 scales it);
+    user_loop.F90:122-150, r0 = r_L0*L0 as user_uniform.F90:38 scales it);
     beta_twist is set so that the centre of the footpoint turns by
     twist_angle (seeded in [pi/2, 3pi/2]) over `steps` steps."""
     nn = (n, n, n)

@@ -1,0 +1,68 @@
+#!/usr/bin/env python3
+"""Short, fixed workloads for hardware-counter and kernel-trace runs.
+
+  step  -- a few fused timesteps of the Alfvenic benchmark workload
+           (B0 = z + seeded dB/B0 = 0.05, k <= 8, periodic) at one precision;
+  copy  -- tff_init_rk_step on the same grid: two launches of a plain copy
+           kernel (k_copy3), E -> En and B -> Bn over the allocated box,
+           3 * nalloc words each way per launch -- a kernel of known
+           traffic, against which FETCH_SIZE / WRITE_SIZE are calibrated.
+
+Each mode prints what it ran (sizes, tile, bytes) so that a counter summary
+can be matched to it.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+import problems  # noqa: E402
+import pytff  # noqa: E402
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("mode", choices=("step", "copy"))
+    ap.add_argument("--precision", default="fp32_bulk", choices=("fp64", "fp32", "fp32_bulk"))
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--tile", default="", help="tile:kchunk (default: autotuned)")
+    a = ap.parse_args(argv)
+    p = problems.alfven(a.n, seed=1, kmax=8)
+    mb = pytff.Meshblock(p.n, boundary=p.boundary, cc=p.cc, precision=a.precision)
+    problems.load(mb, p)
+    mb.preloop()
+    if a.mode == "copy":
+        mb.initRKstep()  # allocates En, Bn
+        mb.sync()
+        for _ in range(a.steps):
+            mb.initRKstep()
+        mb.sync()
+        word = 8 if a.precision == "fp64" else 4
+        nalloc = mb.nalloc
+        print(f"copy: {a.steps} calls, 2 k_copy3 launches each (E, then B), nalloc {nalloc}, "
+              f"{3 * nalloc * word} bytes read and {3 * nalloc * word} written per launch", flush=True)
+        mb.close()
+        return 0
+    if a.tile:
+        t, kc = a.tile.split(":")
+        mb.set_tuning(int(t), int(kc))
+    mb.run(0, 2)
+    mb.sync()
+    t0 = time.perf_counter()
+    mb.run(2, a.steps)
+    mb.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / max(a.steps, 1)
+    print(f"step: {a.precision} {a.n}^3, tuning {mb.get_tuning()}, {a.steps} steps, "
+          f"{ms:.3f} ms/step", flush=True)
+    mb.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -105,6 +105,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.tff_set_path.argtypes = [P, ctypes.c_int]
     lib.tff_set_tuning.argtypes = [P, ctypes.c_int, ctypes.c_int]
     lib.tff_get_tuning.argtypes = [P] + [ctypes.POINTER(ctypes.c_int)] * 4
+    lib.tff_get_layout.argtypes = [P, ctypes.POINTER(ctypes.c_longlong)]
     for name in header_functions():
         if name not in ("tff_destroy", "tff_last_error", "tff_batch_destroy",
                         "tff_batch_last_error", "tff_batch_instance"):
@@ -250,6 +251,15 @@ class Meshblock:
         v = [ctypes.c_int() for _ in range(4)]
         self._check(self.lib.tff_get_tuning(self.h, *[ctypes.byref(x) for x in v]), "tff_get_tuning")
         return {"tile": v[0].value, "tx": v[1].value, "ty": v[2].value, "kchunk": v[3].value}
+
+    def layout(self) -> dict:
+        v = (ctypes.c_longlong * 5)()
+        self._check(self.lib.tff_get_layout(self.h, v), "tff_get_layout")
+        return {"px": v[0], "plane": v[1], "org": v[2], "nalloc": v[3], "esz": v[4]}
+
+    @property
+    def nalloc(self) -> int:
+        return self.layout()["nalloc"]
 
     # ---- timing ---------------------------------------------------------
     def event(self, slot: int):
