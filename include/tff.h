@@ -110,9 +110,13 @@ int tff_download(tff_handle* h, int field, void* host, int host_dtype);
 /* Register the problem's boundary kernel and its parameters (the values the
  * user file's userReadInput computes), in this order:
  *  TFF_BC_NONE     none
- *  TFF_BC_LOOP     xc, yc1, yc2, zc, bnorm, r0, beta_twist, t0, twist
- *                  (user_loop.F90:27-42 and :101-164; twist != 0 turns on
- *                  the footpoint driver of :122-150)
+ *  TFF_BC_LOOP     xc, yc1, yc2, zc, bnorm, r0, beta_twist, t0, mode
+ *                  (user_loop.F90:27-42 and :101-164).  mode 0: the
+ *                  floor/top copy as shipped; 1: the footpoint driver of the
+ *                  commented block :122-150 on the floor ghosts; 2: that
+ *                  driver on z <= 0 (floor plane included) with a conducting
+ *                  top, the boundary form of user_uniform.F90:113-142
+ *                  (single meshblocks only)
  *  TFF_BC_PSR      psr_radius, psr_angle [rad], psr_omega, xc, yc, zc
  *                  (user_psr.F90:26-37 and :72-332)
  *  TFF_BC_UNIFORM  xc, yc, r0, beta_twist, t0, bnorm

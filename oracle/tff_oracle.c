@@ -379,15 +379,20 @@ void orc_exchange_fields(orc_t *o, int exchange_b0) {
 
 /* --- user/user_loop.F90:101-164  userFieldBoundaryConditions ------------
  * bcp: 0 xc, 1 yc1, 2 yc2, 3 zc, 4 bnorm, 5 r0, 6 beta_twist, 7 t0,
- *      8 twist (0: as shipped, floor copy only; 1: the commented-out
- *        footpoint-rotation driver of user_loop.F90:122-150, with the
- *        tanh(time/t0) ramp switched on). */
+ *      8 twist (0: as shipped, floor/top copy; 1: the commented-out
+ *        footpoint-rotation driver of user_loop.F90:122-150 on the floor
+ *        ghosts, with the tanh(time/t0) ramp switched on; 2: the driven loop
+ *        described in the function). */
 static void orc_bc_loop(orc_t *o, int step) {
   real *ex = o->f[F_EX], *ey = o->f[F_EY], *bz = o->f[F_BZ];
   const real xc = (real)o->bcp[0], yc1 = (real)o->bcp[1], yc2 = (real)o->bcp[2];
   const real zc = (real)o->bcp[3], bnorm = (real)o->bcp[4], r0 = (real)o->bcp[5];
   const real beta = (real)o->bcp[6], t0 = (real)o->bcp[7];
-  const int twist = o->bcp[8] != 0.0;
+  /* twist 0: floor/top copy as shipped; 1: the commented footpoint driver on
+   * the floor ghosts (z < 0); 2: builder-defined driven loop in the form of
+   * user_uniform.F90:113-142 -- the driver on z <= 0 (the floor plane
+   * included) and a conducting top (ex = ey = 0, bz = the potential field) */
+  const int twist = (int)o->bcp[8];
   const real time = (real)step;
   const real CC = o->cc;
   for (int i = -NG; i <= o->nx - 1 + NG; ++i) {
@@ -397,7 +402,7 @@ static void orc_bc_loop(orc_t *o, int step) {
       for (int k = -NG; k <= o->nz - 1 + NG; ++k) {
         const real zg = (real)(k + o->z0);
         const int64_t c = IDX(o, i, j, k);
-        if (zg < (real)0) {
+        if (zg < (real)0 || (twist == 2 && zg <= (real)0)) {
           if (twist) {
             real rx, ry1, ry2, rz, rr1, rr2, bzl, omega, ee0;
             rx = xg + (real)0.5 - xc; ry1 = yg - yc1; ry2 = yg - yc2; rz = zg - zc;
@@ -425,9 +430,19 @@ static void orc_bc_loop(orc_t *o, int step) {
           }
         }
         if (zg >= (real)o->gz) {
-          ex[c] = ex[IDX(o, i, j, o->nz - 1)];
-          ey[c] = ey[IDX(o, i, j, o->nz - 1)];
-          bz[c] = bz[IDX(o, i, j, o->nz - 1)];
+          if (twist == 2) {
+            const real rx = xg + (real)0.5 - xc, ry1 = yg + (real)0.5 - yc1;
+            const real ry2 = yg + (real)0.5 - yc2, rz = zg - zc;
+            const real rr1 = (real)sqrt(rx * rx + ry1 * ry1 + rz * rz);
+            const real rr2 = (real)sqrt(rx * rx + ry2 * ry2 + rz * rz);
+            bz[c] = bnorm * (rz / cube(rr1) - rz / cube(rr2));
+            ex[c] = (real)0;
+            ey[c] = (real)0;
+          } else {
+            ex[c] = ex[IDX(o, i, j, o->nz - 1)];
+            ey[c] = ey[IDX(o, i, j, o->nz - 1)];
+            bz[c] = bz[IDX(o, i, j, o->nz - 1)];
+          }
         }
       }
     }

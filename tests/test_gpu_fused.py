@@ -17,6 +17,7 @@ CASES = {
     "emwave_odd": lambda: problems.emwave(36, mode=(2, 1)),
     "loop": lambda: problems.loop(40, twist_angle=1.0, steps=12),
     "loop_shipped": lambda: problems.loop(36, twist=False, steps=12),
+    "loop_driven": lambda: problems.loop(36, twist_angle=1.0, steps=12, driver=problems.LOOP_DRIVEN),
     "psr": lambda: problems.psr(40, radius=5.0, r_lc=12.0, absorb_buff=6, periods=0.2),
     "psr_faithful": lambda: problems.psr(40, radius=5.0, r_lc=12.0, absorb_buff=6, periods=0.2,
                                          absorb_fixed=False),

@@ -102,6 +102,7 @@ SMALL = {
     "alfven": lambda: problems.alfven(24, seed=5, kmax=4, steps=20),
     "loop": lambda: problems.loop(32, steps=12, twist_angle=1.0),
     "loop_shipped": lambda: problems.loop(32, steps=12, twist=False),
+    "loop_driven": lambda: problems.loop(32, steps=12, twist_angle=1.0, driver=problems.LOOP_DRIVEN),
     "psr": lambda: problems.psr(40, radius=5.0, r_lc=12.0, absorb_buff=6, periods=0.2),
     "psr_faithful": lambda: problems.psr(40, radius=5.0, r_lc=12.0, absorb_buff=6, periods=0.2,
                                          absorb_fixed=False),

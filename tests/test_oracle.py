@@ -214,6 +214,48 @@ def test_loop_hook_floor_copy():
     assert np.array_equal(o.field("ez"), ez0)
 
 
+def test_loop_driven_hook_closed_form():
+    """Driven loop: on z <= 0 (the floor plane included) Ex is the footpoint
+    driver -(x+1/2-xc) e^{-r1/r0} beta CC/r0 Bz_loc tanh(t/t0) with r1 the
+    3D distance to source 1 and Bz_loc the potential field there; the top
+    ghost planes are a conductor (Ex = Ey = 0, Bz = the potential field);
+    interior planes above the floor and Ez are untouched."""
+    n = 16
+    p = problems.loop(n, twist_angle=1.0, steps=40, driver=problems.LOOP_DRIVEN)
+    o = Oracle(p.n, boundary=p.boundary, cc=p.cc)
+    problems.load(o, p)
+    rng = np.random.default_rng(2)
+    for f in ("ex", "ey", "ez"):
+        o.field(f)[...] = rng.standard_normal(o.shape)
+    before = {f: o.field(f).copy() for f in ("ex", "ey", "ez", "bz")}
+    step = 9
+    o.user_bc(step)
+    xc, yc1, yc2, zc, bnorm, r0, beta, t0, mode = p.bc_params
+    assert mode == problems.LOOP_DRIVEN
+    i = np.arange(-NG, n + NG)[None, :].astype(float)
+    j = np.arange(-NG, n + NG)[:, None].astype(float)
+
+    def bz_pot(rx, ry1, ry2, rz):
+        r1 = np.sqrt(rx * rx + ry1 * ry1 + rz * rz)
+        r2 = np.sqrt(rx * rx + ry2 * ry2 + rz * rz)
+        return bnorm * (rz / r1 ** 3 - rz / r2 ** 3), r1
+
+    for kk, z in ((0, -2.0), (1, -1.0), (NG, 0.0)):
+        rx, ry1, ry2, rz = i + 0.5 - xc, j - yc1, j - yc2, z - zc
+        bzl, r1 = bz_pot(rx, ry1, ry2, rz)
+        ex = -rx * np.exp(-r1 / r0) * beta * p.cc / r0 * bzl * math.tanh(step / t0)
+        assert np.allclose(o.field("ex")[kk], ex, rtol=1e-12, atol=1e-300)
+        bz, _ = bz_pot(i + 0.5 - xc, j + 0.5 - yc1, j + 0.5 - yc2, z - zc)
+        assert np.allclose(o.field("bz")[kk], bz, rtol=1e-12)
+    for kk, z in ((-2, float(n)), (-1, float(n + 1))):
+        assert not o.field("ex")[kk].any() and not o.field("ey")[kk].any()
+        bz, _ = bz_pot(i + 0.5 - xc, j + 0.5 - yc1, j + 0.5 - yc2, z - zc)
+        assert np.allclose(o.field("bz")[kk], bz, rtol=1e-12)
+    for f in ("ex", "ey", "bz"):
+        assert np.array_equal(o.field(f)[NG + 1:-NG], before[f][NG + 1:-NG])
+    assert np.array_equal(o.field("ez"), before["ez"])
+
+
 def test_uniform_hook_closed_form():
     """user_uniform: Ex = -(x+1/2-xc) e^{-r/r0} beta CC/r0 bnorm tanh(t/t0) on z <= 0."""
     p = problems.uniform(16, L0=4.0, beta_twist=0.3, t0=5.0)

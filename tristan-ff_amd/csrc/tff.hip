@@ -220,7 +220,7 @@ int launch_bc(tff_handle* h, const Vec& e, const Vec& b, const int* step_ptr, in
   const P cc = std::is_same<S, P>::value ? (P)(S)h->cfg.cc : (P)h->cfg.cc;
   switch (h->bc_kind) {
     case TFF_BC_LOOP: {
-      const int64_t n = (int64_t)2 * NG * sx * sy;
+      const int64_t n = (int64_t)(2 * NG + 1) * sx * sy;
       hipLaunchKernelGGL(k_bc_loop<PR>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, h->stream,
                          h->g, mv3<S>(e), mv3<S>(b), h->d_bc, cc, step_ptr, step_val);
       return last_launch(h, "k_bc_loop");
@@ -1451,6 +1451,12 @@ int tff_batch_create(const tff_instance* in, int n, int precision, int device, t
   int rc = TFF_OK;
   for (int q = 0; q < n && !rc; ++q) {
     if (in[q].bc_kind != TFF_BC_NONE && in[q].bc_kind != TFF_BC_LOOP) {
+      rc = TFF_EINVAL;
+      break;
+    }
+    // the batched ghost passes write ghost cells only: the driven-loop mode,
+    // which also writes the interior floor plane, runs on single meshblocks
+    if (in[q].bc_kind == TFF_BC_LOOP && (in[q].nparams < 9 || in[q].params[8] > 1.0)) {
       rc = TFF_EINVAL;
       break;
     }

@@ -40,16 +40,26 @@ __device__ __forceinline__ int find_item(const int64_t* base, int n, int64_t t) 
   return lo;
 }
 
-// Persistent workgroups walk the (instance, tile) list with a fixed stride:
-// workgroup w takes items w, w + G, w + 2G, ... (G = grid size), so every
-// item is done exactly once with no cross-workgroup synchronisation, and
-// the loop ends for every workgroup when its next item is past the end.
+// Persistent workgroups walk the (instance, tile) list with fixed strides,
+// so every item is done exactly once with no cross-workgroup
+// synchronisation and every workgroup leaves when its next item is past the
+// end.  Workgroups are dealt to the 8 XCDs round-robin (blockIdx % 8); when
+// the grid is a multiple of 8, XCD x works through the x-th contiguous
+// eighth of the list, so tiles that share halo rows share that XCD's L2.
 template <class PR, class TL>
 __global__ void __launch_bounds__(TL::NT)
-    k_batch_stage(const BatchStage<typename PR::S, typename PR::P>* st, const int64_t* tile_base, int ninst,
-                  int64_t total) {
+    k_batch_stage(const BatchStage<typename PR::S, typename PR::P>* __restrict__ st,
+                  const int64_t* __restrict__ tile_base, int ninst, int64_t total) {
   extern __shared__ unsigned char smem_raw[];
-  for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+  int64_t t0 = blockIdx.x, stride = gridDim.x, end = total;
+  if ((gridDim.x & 7) == 0) {
+    const int64_t chunk = (total + 7) / 8;
+    const int64_t x = blockIdx.x & 7;
+    t0 = x * chunk + (blockIdx.x >> 3);
+    stride = gridDim.x >> 3;
+    end = min(total, (x + 1) * chunk);
+  }
+  for (int64_t t = t0; t < end; t += stride) {
     __syncthreads();  // the previous item has finished with LDS
     const int inst = find_item(tile_base, ninst, t);
     stage_tile<PR, TL>(st[inst].a, (int)(t - tile_base[inst]), smem_raw);

@@ -197,9 +197,22 @@ __global__ void __launch_bounds__(BX* BY)
 // Boundary hooks.
 
 // user_loop.F90:101-164.  p: 0 xc, 1 yc1, 2 yc2, 3 zc, 4 bnorm, 5 r0,
-// 6 beta_twist, 7 t0, 8 twist flag (0: floor/top copy as shipped; 1: the
-// footpoint-rotation driver of the commented block :122-150 with its
-// tanh(time/t0) ramp on).  Only z-ghost planes are written.
+// 6 beta_twist, 7 t0, 8 mode:
+//   0  the floor/top copy as shipped (z-ghost planes only);
+//   1  the footpoint-rotation driver of the commented block :122-150, with
+//      its tanh(time/t0) ramp on, on the floor ghosts; top copy;
+//   2  a driven loop in the boundary form of user_uniform.F90:113-142: the
+//      same driver on z <= 0 (the interior floor plane included) and a
+//      conducting top (ex = ey = 0, bz = the potential field).
+// One thread per (i, j) of the ghost-inclusive plane and per plane of
+// k = -2, -1, 0, nz, nz+1; plane 0 is written in mode 2 only.
+template <class P>
+__device__ __forceinline__ P loop_bz(P bnorm, P rx, P ry1, P ry2, P rz) {
+  const P rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
+  const P rr2 = sqrt(rx * rx + ry2 * ry2 + rz * rz);
+  return bnorm * (rz / (rr1 * rr1 * rr1) - rz / (rr2 * rr2 * rr2));
+}
+
 template <class PR>
 __global__ void k_bc_loop(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, const double* p,
                           typename PR::P cc, const int* step_ptr, int step_val) {
@@ -207,43 +220,46 @@ __global__ void k_bc_loop(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, co
   using P = typename PR::P;
   const int sx = g.nx + 2 * NG, sy = g.ny + 2 * NG;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)2 * NG * sx * sy) return;
+  if (t >= (int64_t)(2 * NG + 1) * sx * sy) return;
   const int i = (int)(t % sx) - NG;
   const int j = (int)((t / sx) % sy) - NG;
   const int kk = (int)(t / ((int64_t)sx * sy));
-  const int k = kk < NG ? kk - NG : g.nz + kk - NG;
+  const int k = kk < NG ? kk - NG : (kk == NG ? 0 : g.nz + kk - NG - 1);
+  const int mode = (int)p[8];
+  if (k == 0 && mode != 2) return;
   const int64_t c = g.at(i, j, k);
-  if (k < 0) {
-    if (p[8] != 0.0) {
+  const P xc = (P)p[0], yc1 = (P)p[1], yc2 = (P)p[2], zc = (P)p[3], bnorm = (P)p[4];
+  const P xg = (P)i, yg = (P)j, zg = (P)k, hf = (P)0.5;
+  if (k <= 0) {
+    if (mode != 0) {
       const int step = step_ptr ? *step_ptr : step_val;
-      const P xc = (P)p[0], yc1 = (P)p[1], yc2 = (P)p[2], zc = (P)p[3], bnorm = (P)p[4];
       const P r0 = (P)p[5], beta = (P)p[6], t0 = (P)p[7];
-      const P xg = (P)i, yg = (P)j, zg = (P)k, time = (P)step, hf = (P)0.5;
-      P rx, ry1, ry2, rz, rr1, rr2, bzl, omega, ee0;
+      const P time = (P)step;
+      P rx, ry1, ry2, rz, rr1, bzl, omega, ee0;
       rx = xg + hf - xc; ry1 = yg - yc1; ry2 = yg - yc2; rz = zg - zc;
       rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
-      rr2 = sqrt(rx * rx + ry2 * ry2 + rz * rz);
-      bzl = bnorm * (rz / (rr1 * rr1 * rr1) - rz / (rr2 * rr2 * rr2));
+      bzl = loop_bz(bnorm, rx, ry1, ry2, rz);
       omega = exp(-rr1 / r0) * beta * cc / r0;
       ee0 = -omega * bzl * tanh(time / t0);
       e.x[c] = (S)(rx * ee0);
       rx = xg - xc; ry1 = yg + hf - yc1; ry2 = yg + hf - yc2; rz = zg - zc;
       rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
-      rr2 = sqrt(rx * rx + ry2 * ry2 + rz * rz);
-      bzl = bnorm * (rz / (rr1 * rr1 * rr1) - rz / (rr2 * rr2 * rr2));
+      bzl = loop_bz(bnorm, rx, ry1, ry2, rz);
       omega = exp(-rr1 / r0) * beta * cc / r0;
       ee0 = -omega * bzl * tanh(time / t0);
       e.y[c] = (S)(ry1 * ee0);
       rx = xg + hf - xc; ry1 = yg + hf - yc1; ry2 = yg + hf - yc2; rz = zg - zc;
-      rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
-      rr2 = sqrt(rx * rx + ry2 * ry2 + rz * rz);
-      b.z[c] = (S)(bnorm * (rz / (rr1 * rr1 * rr1) - rz / (rr2 * rr2 * rr2)));
+      b.z[c] = (S)loop_bz(bnorm, rx, ry1, ry2, rz);
     } else {
       const int64_t s = g.at(i, j, 0);
       e.x[c] = e.x[s];
       e.y[c] = e.y[s];
       b.z[c] = b.z[s];
     }
+  } else if (mode == 2) {
+    b.z[c] = (S)loop_bz(bnorm, xg + hf - xc, yg + hf - yc1, yg + hf - yc2, zg - zc);
+    e.x[c] = (S)0;
+    e.y[c] = (S)0;
   } else {
     const int64_t s = g.at(i, j, g.nz - 1);
     e.x[c] = e.x[s];

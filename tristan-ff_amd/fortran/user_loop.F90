@@ -1,8 +1,9 @@
 ! user_loop.F90 -- two opposite magnetic sources below a conducting floor
 ! (user/user_loop.F90:27-93), x and y periodic, z open.  The floor/top
-! hook runs on the device (TFF_BC_LOOP); with <problem> twist = 1 the floor
-! also rotates footpoint 1 (the driver the reference leaves commented out
-! at user_loop.F90:122-150, with r0 = r_L0 * L0 and its tanh(t/t0) ramp).
+! hook runs on the device (TFF_BC_LOOP); <problem> twist = 1 adds the
+! footpoint-1 rotation that the reference leaves commented out at
+! user_loop.F90:122-150 (floor ghosts, r0 = r_L0 * L0, tanh(t/t0) ramp);
+! twist = 2 applies it on z <= 0 with a conducting top instead.
 #define NGHOST 2
 module m_userfile
   use m_globalnamespace

@@ -151,8 +151,11 @@ def alfven(n=256, seed=1, amp=0.05, kmax=8, cc=0.45, steps=500) -> Problem:
 
 
 # --- config: twisted loop (user_loop.F90 form) -----------------------------
+LOOP_COPY, LOOP_GHOST_DRIVER, LOOP_DRIVEN = 0, 1, 2
+
+
 def loop(n=384, L0=None, a_L0=3.0, h_L0=2.0, r_L0=0.5, cc=0.45, twist_angle=None, seed=3,
-         t0=None, steps=1000, twist=True, dtype=np.float64) -> Problem:
+         t0=None, steps=1000, twist=True, driver=LOOP_GHOST_DRIVER, dtype=np.float64) -> Problem:
     """user_loop.F90:27-93: two opposite sources at depth h*L0 below the
     floor, +-a*L0 either side of the box centre; B0 = B everywhere including
     ghosts; x, y periodic, z open with the floor/top hook.  L0 scales as
@@ -160,7 +163,13 @@ def loop(n=384, L0=None, a_L0=3.0, h_L0=2.0, r_L0=0.5, cc=0.45, twist_angle=None
     floor hook also rotates footpoint 1 (the driver of the commented block
     user_loop.F90:122-150, r0 = r_L0*L0 as user_uniform.F90:38 scales it);
     beta_twist is set so that the centre of the footpoint turns by
-    twist_angle (seeded in [pi/2, 3pi/2]) over `steps` steps."""
+    twist_angle (seeded in [pi/2, 3pi/2]) over `steps` steps.
+
+    driver = LOOP_DRIVEN instead applies that driver on z <= 0 (the floor
+    plane included) and closes the top as a conductor (ex = ey = 0, bz = the
+    potential field), the boundary form of user_uniform.F90:113-142: the
+    ghost-only driver and the floor/top copy never reach the interior
+    update, so only this form twists the loop."""
     nn = (n, n, n)
     d = np.dtype(dtype).type
     L0 = 20.0 * n / 256 if L0 is None else L0
@@ -192,10 +201,11 @@ def loop(n=384, L0=None, a_L0=3.0, h_L0=2.0, r_L0=0.5, cc=0.45, twist_angle=None
     for c in "xyz":
         f["b0" + c][...] = f["b" + c]
     # footpoint angular speed is CC * omega (drift v = c E x B / B^2, E = -omega r x B)
-    dist = float(h0 * L0) - 1.0
+    dist = float(h0 * L0) - (0.0 if driver == LOOP_DRIVEN else 1.0)
     ramp = sum(math.tanh(t / t0) for t in range(steps))
     beta = twist_angle * float(r0) * math.exp(dist / float(r0)) / (cc * cc * max(ramp, 1e-30))
-    params = (xcf, yc1f, yc2f, zcf, float(bnorm), float(r0), beta, float(t0), 1.0 if twist else 0.0)
+    mode = float(driver) if twist else float(LOOP_COPY)
+    params = (xcf, yc1f, yc2f, zcf, float(bnorm), float(r0), beta, float(t0), mode)
     return Problem("loop", nn, (1, 1, 0), cc, f, BC_LOOP, params, steps=steps, g=1)
 
 

@@ -7,8 +7,9 @@ the FP32-bulk path's error against the FP64 path on the same steps.
              solution;
   alfven256  B0 = z + seeded Alfvenic dB/B0 = 0.05 (k <= 8), 256^3 periodic,
              CC = 0.45, 500 steps;
-  loop384    twisted loop (user_loop form, footpoint twist driver, seeded
-             total angle), 384^3, x/y periodic, z floor, 1000 steps;
+  loop384    twisted loop (user_loop field; the footpoint driver on z <= 0
+             and a conducting top, seeded total angle), 384^3, x/y periodic,
+             z open, 1000 steps;
   psr512     aligned rotator (user_psr form), R* = 20, R_LC = 80, absorbing
              layer 40 cells (fixed mode), 512^3, CC = 0.5, two periods
              (about 2011 steps);
@@ -224,7 +225,7 @@ def main(argv=None) -> int:
             res = single_workload(w, problems.alfven(256, seed=1, kmax=8), steps(500), 50)
         elif w == "loop384":
             ns = steps(1000)
-            res = single_workload(w, problems.loop(384, steps=ns), ns, 100)
+            res = single_workload(w, problems.loop(384, steps=ns, driver=problems.LOOP_DRIVEN), ns, 100)
         elif w == "psr512":
             p = problems.psr(512)
             ns = steps(p.steps)
