@@ -186,6 +186,25 @@ def test_fp32_bulk_energy_vs_fp64_oracle():
         assert normwise_max_rel(g.download(f), o.field(f)) < 1e-4
 
 
+@pytest.mark.parametrize("precision", ["fp32_bulk", "fp32"])
+def test_psr_low_precision_finite_and_close_to_fp64(precision):
+    """The rotator with the fixed absorbing layer: its corners beyond r_max
+    are zeroed (B = 0) while E arrives there at underflowing amplitudes, the
+    case that an |E|^2 reciprocal overflowing to +inf turns into 0 * inf =
+    NaN.  Every field must stay finite and the energy close to the
+    oracle's (FP64 for FP32-bulk, FP32 for FP32)."""
+    p = problems.psr(64, radius=5.0, r_lc=20.0, absorb_buff=8)
+    nsteps = 60
+    o, g = _run_both(p, 0, precision=precision)
+    o.run(0, nsteps)
+    g.run(0, nsteps)
+    for f in FIELDS6:
+        assert np.isfinite(g.download(f)).all(), f
+    eo, eg = sum(o.reduce()[:2]), sum(g.reduce()[:2])
+    print(f"psr {precision}: energy {eg:.9e} vs oracle {eo:.9e}")
+    assert abs(eg - eo) <= 1e-5 * eo
+
+
 def test_reduce_matches_oracle():
     p = problems.alfven(24, seed=9, kmax=4)
     o, g = _run_both(p, 3)

@@ -1098,6 +1098,10 @@ struct tff_batch {
   int64_t* d_tile_base = nullptr;
   int64_t* d_shell_base = nullptr;
   int* d_shell_inst = nullptr;
+  int64_t* d_floor_base = nullptr;  // driven-loop instances: interior floor planes
+  int* d_floor_inst = nullptr;
+  int nfloor = 0;
+  int64_t total_floor = 0;
   void* d_info = nullptr;
   void* d_stage = nullptr;  // BatchStage[4][n]: stage 1 from a clean state, stages 1, 2, 3
   int* d_step = nullptr;
@@ -1185,10 +1189,10 @@ int batch_build_t(tff_batch* b) {
   using P = typename PR::P;
   const int n = (int)b->inst.size();
   std::vector<BatchStage<S, P>> st(4 * (size_t)n);
-  std::vector<int64_t> tile_base(n), shell_base;
-  std::vector<int> shell_inst;
+  std::vector<int64_t> tile_base(n), shell_base, floor_base;
+  std::vector<int> shell_inst, floor_inst;
   std::vector<BatchInfo> info(n);
-  int64_t tt = 0, ss = 0;
+  int64_t tt = 0, ss = 0, ff = 0;
   for (int q = 0; q < n; ++q) {
     tff_handle* h = b->inst[q];
     h->kchunk = b->kchunk;
@@ -1203,6 +1207,11 @@ int batch_build_t(tff_batch* b) {
       shell_base.push_back(ss);
       shell_inst.push_back(q);
       ss += shell_count(h);
+    }
+    if (h->bc_kind == TFF_BC_LOOP && h->bc_host[8] == 2.0) {
+      floor_base.push_back(ff);
+      floor_inst.push_back(q);
+      ff += (int64_t)h->g.nx * h->g.ny;
     }
     for (int w = 0; w < 4; ++w) {
       BatchStage<S, P>& x = st[(size_t)w * n + q];
@@ -1258,6 +1267,8 @@ int batch_build_t(tff_batch* b) {
   b->total_tiles = tt;
   b->total_shell = ss;
   b->nghost = (int)shell_base.size();
+  b->nfloor = (int)floor_base.size();
+  b->total_floor = ff;
   // the batch stream is non-blocking: the old lists may only be freed, and
   // the new ones filled, in its order
   BHIP(b, hipStreamSynchronize(b->stream));
@@ -1265,7 +1276,11 @@ int batch_build_t(tff_batch* b) {
   hipFree(b->d_tile_base);
   hipFree(b->d_shell_base);
   hipFree(b->d_shell_inst);
+  hipFree(b->d_floor_base);
+  hipFree(b->d_floor_inst);
   hipFree(b->d_info);
+  b->d_floor_base = nullptr;
+  b->d_floor_inst = nullptr;
   b->d_stage = b->d_tile_base = nullptr;
   b->d_shell_base = nullptr;
   b->d_shell_inst = nullptr;
@@ -1281,6 +1296,14 @@ int batch_build_t(tff_batch* b) {
     BHIP(b, hipMemcpyAsync(b->d_shell_base, shell_base.data(), shell_base.size() * sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipMalloc((void**)&b->d_shell_inst, shell_inst.size() * sizeof(int)));
     BHIP(b, hipMemcpyAsync(b->d_shell_inst, shell_inst.data(), shell_inst.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+  }
+  if (b->nfloor) {
+    BHIP(b, hipMalloc((void**)&b->d_floor_base, floor_base.size() * sizeof(int64_t)));
+    BHIP(b, hipMemcpyAsync(b->d_floor_base, floor_base.data(), floor_base.size() * sizeof(int64_t),
+                           hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMalloc((void**)&b->d_floor_inst, floor_inst.size() * sizeof(int)));
+    BHIP(b, hipMemcpyAsync(b->d_floor_inst, floor_inst.data(), floor_inst.size() * sizeof(int),
+                           hipMemcpyHostToDevice, b->stream));
   }
   BHIP(b, hipStreamSynchronize(b->stream));  // the host vectors are in use until here
   return batch_grid<PR>(b);
@@ -1304,6 +1327,10 @@ int batch_ghosts(tff_batch* b, int which) {
   const int n = (int)b->inst.size();
   const BatchStage<S, P>* st = (const BatchStage<S, P>*)b->d_stage + (size_t)which * n;
   const int tb = 256;
+  if (b->nfloor)
+    hipLaunchKernelGGL(k_batch_floor<PR>, dim3((unsigned)((b->total_floor + tb - 1) / tb)), dim3(tb), 0, b->stream,
+                       st, (const BatchInfo*)b->d_info, b->d_floor_base, b->d_floor_inst, b->nfloor,
+                       b->total_floor, b->d_step);
   const unsigned nb = (unsigned)((b->total_shell + tb - 1) / tb);
   hipLaunchKernelGGL(k_batch_ghost_build<PR>, dim3(nb), dim3(tb), 0, b->stream, st, (const BatchInfo*)b->d_info,
                      b->d_shell_base, b->d_shell_inst, b->nghost, b->total_shell, b->d_step);
@@ -1454,9 +1481,7 @@ int tff_batch_create(const tff_instance* in, int n, int precision, int device, t
       rc = TFF_EINVAL;
       break;
     }
-    // the batched ghost passes write ghost cells only: the driven-loop mode,
-    // which also writes the interior floor plane, runs on single meshblocks
-    if (in[q].bc_kind == TFF_BC_LOOP && (in[q].nparams < 9 || in[q].params[8] > 1.0)) {
+    if (in[q].bc_kind == TFF_BC_LOOP && (in[q].nparams < 9 || in[q].params[8] < 0.0 || in[q].params[8] > 2.0)) {
       rc = TFF_EINVAL;
       break;
     }
@@ -1496,6 +1521,8 @@ void tff_batch_destroy(tff_batch* b) {
   hipFree(b->d_tile_base);
   hipFree(b->d_shell_base);
   hipFree(b->d_shell_inst);
+  hipFree(b->d_floor_base);
+  hipFree(b->d_floor_inst);
   hipFree(b->d_info);
   hipFree(b->d_stage);
   hipFree(b->d_step);

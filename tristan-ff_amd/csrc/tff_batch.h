@@ -7,7 +7,9 @@
 // Instances with stored ghosts then get their ghost cells in two batched
 // shell passes: the new ghost values (what rk3Update leaves, the floor/top
 // hook of user_loop.F90:101-164, and exchangeFields' periodic fill, applied
-// in the reference's order) and the cleaned E of every ghost cell.
+// in the reference's order) and the cleaned E of every ghost cell.  A
+// driven loop's hook also writes its interior floor plane: one more batched
+// pass, before the ghost values.
 #pragma once
 #include "tff_fused.h"
 
@@ -47,7 +49,7 @@ __device__ __forceinline__ int find_item(const int64_t* base, int n, int64_t t) 
 // the grid is a multiple of 8, XCD x works through the x-th contiguous
 // eighth of the list, so tiles that share halo rows share that XCD's L2.
 template <class PR, class TL>
-__global__ void __launch_bounds__(TL::NT)
+__global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(sizeof(typename PR::S) == 4 ? 3 : 2)))
     k_batch_stage(const BatchStage<typename PR::S, typename PR::P>* __restrict__ st,
                   const int64_t* __restrict__ tile_base, int ninst, int64_t total) {
   extern __shared__ unsigned char smem_raw[];
@@ -106,52 +108,68 @@ __global__ void k_batch_ghost_build(const BatchStage<typename PR::S, typename PR
     e3[c] = b.ge_in[d + c * cs];
     b3[c] = b.prev_b[d + c * cs];
   }
-  if (in.bc_kind == 1 && oz) {
-    // user_loop floor/top: ex, ey, bz take the k = 0 (or nz-1) value the
-    // column holds before the fill -- the new interior value, or for a ghost
-    // column the value carried from the stage input
-    const int ks = k < 0 ? 0 : g.nz - 1;
-    const int64_t s = g.at(i, j, ks);
-    const bool col_inside = !ox && !oy;
-    if (k < 0 && in.bcp[8] != 0.0) {
-      const double* p = in.bcp;
-      const int step = *step_ptr;
-      const P xc = (P)p[0], yc1 = (P)p[1], yc2 = (P)p[2], zc = (P)p[3], bnorm = (P)p[4];
-      const P r0 = (P)p[5], beta = (P)p[6], t0 = (P)p[7], cc = b.a.cc;
-      const P xg = (P)i, yg = (P)j, zg = (P)k, time = (P)step, hf = (P)0.5;
-      P rx, ry1, ry2, rz, rr1, rr2, bzl, omega, ee0;
-      rx = xg + hf - xc; ry1 = yg - yc1; ry2 = yg - yc2; rz = zg - zc;
-      rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
-      rr2 = sqrt(rx * rx + ry2 * ry2 + rz * rz);
-      bzl = bnorm * (rz / (rr1 * rr1 * rr1) - rz / (rr2 * rr2 * rr2));
-      omega = exp(-rr1 / r0) * beta * cc / r0;
-      ee0 = -omega * bzl * tanh(time / t0);
-      e3[0] = (S)(rx * ee0);
-      rx = xg - xc; ry1 = yg + hf - yc1; ry2 = yg + hf - yc2; rz = zg - zc;
-      rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
-      rr2 = sqrt(rx * rx + ry2 * ry2 + rz * rz);
-      bzl = bnorm * (rz / (rr1 * rr1 * rr1) - rz / (rr2 * rr2 * rr2));
-      omega = exp(-rr1 / r0) * beta * cc / r0;
-      ee0 = -omega * bzl * tanh(time / t0);
-      e3[1] = (S)(ry1 * ee0);
-      rx = xg + hf - xc; ry1 = yg + hf - yc1; ry2 = yg + hf - yc2; rz = zg - zc;
-      rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
-      rr2 = sqrt(rx * rx + ry2 * ry2 + rz * rz);
-      b3[2] = (S)(bnorm * (rz / (rr1 * rr1 * rr1) - rz / (rr2 * rr2 * rr2)));
-    } else if (col_inside) {
-      e3[0] = oe[s];
-      e3[1] = oe[s + cs];
-      b3[2] = ob[s + 2 * cs];
+  const int mode = in.bc_kind == 1 ? (int)in.bcp[8] : -1;
+  if (mode >= 0 && (oz || (mode == 2 && k == 0))) {
+    // user_loop floor/top (ghost-inclusive planes): ex, ey, bz
+    if (mode != 0 && k <= 0) {
+      // the footpoint driver (and, in mode 2, the ghost columns of the
+      // floor plane, which no periodic fill reaches)
+      P ex, ey, bz;
+      loop_driver<P>(in.bcp, b.a.cc, *step_ptr, i, j, k, ex, ey, bz);
+      e3[0] = (S)ex;
+      e3[1] = (S)ey;
+      b3[2] = (S)bz;
+    } else if (mode == 2) {
+      // conducting top
+      e3[0] = (S)0;
+      e3[1] = (S)0;
+      b3[2] = (S)loop_top_bz<P>(in.bcp, i, j, k);
     } else {
-      e3[0] = b.ge_in[s];
-      e3[1] = b.ge_in[s + cs];
-      b3[2] = b.prev_b[s + 2 * cs];
+      // copy of the k = 0 (or nz-1) value the column holds before the fill:
+      // the new interior value, or for a ghost column the value carried
+      // from the stage input
+      const int ks = k < 0 ? 0 : g.nz - 1;
+      const int64_t s = g.at(i, j, ks);
+      if (!ox && !oy) {
+        e3[0] = oe[s];
+        e3[1] = oe[s + cs];
+        b3[2] = ob[s + 2 * cs];
+      } else {
+        e3[0] = b.ge_in[s];
+        e3[1] = b.ge_in[s + cs];
+        b3[2] = b.prev_b[s + 2 * cs];
+      }
     }
   }
   for (int c = 0; c < 3; ++c) {
     oe[d + c * cs] = e3[c];
     ob[d + c * cs] = b3[c];
   }
+}
+
+// The interior floor plane of every driven-loop instance (mode 2): the
+// footpoint driver's ex, ey, bz at k = 0, written before the ghost pass so
+// that periodic ghost columns copy the driven values.
+template <class PR>
+__global__ void k_batch_floor(const BatchStage<typename PR::S, typename PR::P>* st, const BatchInfo* info,
+                              const int64_t* floor_base, const int* floor_inst, int nfloor, int64_t total,
+                              const int* step_ptr) {
+  using S = typename PR::S;
+  using P = typename PR::P;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int q = find_item(floor_base, nfloor, t);
+  const int inst = floor_inst[q];
+  const BatchStage<S, P>& b = st[inst];
+  const Geom& g = b.a.g;
+  const int64_t r = t - floor_base[q];
+  const int i = (int)(r % g.nx), j = (int)(r / g.nx);
+  P ex, ey, bz;
+  loop_driver<P>(info[inst].bcp, b.a.cc, *step_ptr, i, j, 0, ex, ey, bz);
+  const int64_t c = g.at(i, j, 0);
+  b.a.oe[c] = (S)ex;
+  b.a.oe[c + g.cs] = (S)ey;
+  b.a.ob[c + 2 * g.cs] = (S)bz;
 }
 
 // Cleaned E at every ghost cell of the stage output (k_ghost_clean for the

@@ -9,28 +9,55 @@
 
 namespace tff {
 
+// B interpolated to the three E points and |B|^2 there: index 0 is the Ex
+// point (i+1/2, j, k), 1 the Ey point, 2 the Ez point.  Bx at the Ex point
+// is the 8-point average over i..i+1, j-1..j, k-1..k, By and Bz the 2-point
+// averages along z and y (field_solver.F90:96-101), and cyclically for the
+// other points (:119-124, :142-147).  cleanEpar interpolates B with the same
+// expressions (:288-328), so one evaluation serves both.  Sums in T2,
+// stored in Q2; |B|^2 in the order x, y, z.
+template <class Q2>
+struct BBar {
+  Q2 x[3], y[3], z[3], m[3];
+};
+
+template <class T2, class Q2, bool FAST, class BXa, class BYa, class BZa>
+__device__ __forceinline__ BBar<Q2> bbar(const BXa& BXF, const BYa& BYF, const BZa& BZF) {
+  const T2 h = (T2)0.5, e = (T2)0.125;
+  BBar<Q2> r;
+  r.x[0] = (T2)(e * ((T2)BXF(0, 0, 0) + (T2)BXF(0, -1, 0) + (T2)BXF(1, -1, 0) + (T2)BXF(1, 0, 0) +
+                     (T2)BXF(0, 0, -1) + (T2)BXF(0, -1, -1) + (T2)BXF(1, -1, -1) + (T2)BXF(1, 0, -1)));
+  r.y[0] = (T2)(h * ((T2)BYF(0, 0, 0) + (T2)BYF(0, 0, -1)));
+  r.z[0] = (T2)(h * ((T2)BZF(0, 0, 0) + (T2)BZF(0, -1, 0)));
+  r.x[1] = (T2)(h * ((T2)BXF(0, 0, 0) + (T2)BXF(0, 0, -1)));
+  r.y[1] = (T2)(e * ((T2)BYF(0, 0, 0) + (T2)BYF(-1, 0, 0) + (T2)BYF(-1, 1, 0) + (T2)BYF(0, 1, 0) +
+                     (T2)BYF(0, 0, -1) + (T2)BYF(-1, 0, -1) + (T2)BYF(-1, 1, -1) + (T2)BYF(0, 1, -1)));
+  r.z[1] = (T2)(h * ((T2)BZF(0, 0, 0) + (T2)BZF(-1, 0, 0)));
+  r.x[2] = (T2)(h * ((T2)BXF(0, 0, 0) + (T2)BXF(0, -1, 0)));
+  r.y[2] = (T2)(h * ((T2)BYF(0, 0, 0) + (T2)BYF(-1, 0, 0)));
+  r.z[2] = (T2)(e * ((T2)BZF(0, 0, 0) + (T2)BZF(-1, 0, 0) + (T2)BZF(-1, -1, 0) + (T2)BZF(0, -1, 0) +
+                     (T2)BZF(0, 0, 1) + (T2)BZF(-1, 0, 1) + (T2)BZF(-1, -1, 1) + (T2)BZF(0, -1, 1)));
+#pragma unroll
+  for (int p = 0; p < 3; ++p) r.m[p] = sq3<FAST>(r.x[p], r.y[p], r.z[p]);
+  return r;
+}
+
 // cleanEpar at one cell (field_solver.F90:288-328) with sums in T2 and the
-// projection in Q2: returns the three cleaned E components.  Interpolants:
-// E along its own axis at the point, the other two as 4-point averages, B as
-// one 8-point and two 2-point averages; epar = E.B; E <- E - epar B / |B|^2
-// (no TINY).  With `guard`, |B|^2 == 0 leaves E unchanged.  `near` reports a
-// point whose |B|^2 is below NEAR_NULL times `bref` (when bref > 0).
-template <class T2, class Q2, bool FAST, class AX, class AY, class AZ, class BXa, class BYa, class BZa>
-__device__ __forceinline__ void clean_impl(const AX& EX, const AY& EY, const AZ& EZ, const BXa& BXF,
-                                           const BYa& BYF, const BZa& BZF, int guard, Q2 bref,
-                                           Q2& ox, Q2& oy, Q2& oz, bool& near) {
-  const T2 h = (T2)0.5, q = (T2)0.25, e = (T2)0.125;
+// projection in Q2, given the B interpolants: returns the three cleaned E
+// components.  E along its own axis at the point, the other two as 4-point
+// averages; epar = E.B; E <- E - epar B / |B|^2 (no TINY).  With `guard`,
+// |B|^2 == 0 leaves E unchanged.  `near` reports a point whose |B|^2 is
+// below NEAR_NULL times `bref` (when bref > 0).
+template <class T2, class Q2, bool FAST, class AX, class AY, class AZ>
+__device__ __forceinline__ void clean_impl(const AX& EX, const AY& EY, const AZ& EZ, const BBar<Q2>& bb,
+                                           int guard, Q2 bref, Q2& ox, Q2& oy, Q2& oz, bool& near) {
+  const T2 q = (T2)0.25;
   near = false;
   {
     T2 intex = (T2)EX(0, 0, 0);
     T2 intey = q * ((T2)EY(0, 0, 0) + (T2)EY(1, 0, 0) + (T2)EY(0, -1, 0) + (T2)EY(1, -1, 0));
     T2 intez = q * ((T2)EZ(0, 0, 0) + (T2)EZ(1, 0, 0) + (T2)EZ(0, 0, -1) + (T2)EZ(1, 0, -1));
-    T2 intbx = e * ((T2)BXF(0, 0, 0) + (T2)BXF(0, -1, 0) + (T2)BXF(1, -1, 0) + (T2)BXF(1, 0, 0) +
-                    (T2)BXF(0, 0, -1) + (T2)BXF(0, -1, -1) + (T2)BXF(1, -1, -1) + (T2)BXF(1, 0, -1));
-    T2 intby = h * ((T2)BYF(0, 0, 0) + (T2)BYF(0, 0, -1));
-    T2 intbz = h * ((T2)BZF(0, 0, 0) + (T2)BZF(0, -1, 0));
-    Q2 bx_ = intbx, by_ = intby, bz_ = intbz;
-    Q2 bmag = sq3<FAST>(bx_, by_, bz_);
+    const Q2 bx_ = bb.x[0], by_ = bb.y[0], bz_ = bb.z[0], bmag = bb.m[0];
     Q2 epar = FAST ? mad_<FAST>((Q2)intez, bz_, mad_<FAST>((Q2)intey, by_, (Q2)intex * bx_))
                    : ((Q2)intex * bx_ + (Q2)intey * by_ + (Q2)intez * bz_);
     near |= bmag < (Q2)NEAR_NULL * bref || (FAST && bmag < (Q2)1e-30f);
@@ -41,12 +68,7 @@ __device__ __forceinline__ void clean_impl(const AX& EX, const AY& EY, const AZ&
     T2 intex = q * ((T2)EX(0, 0, 0) + (T2)EX(-1, 0, 0) + (T2)EX(0, 1, 0) + (T2)EX(-1, 1, 0));
     T2 intey = (T2)EY(0, 0, 0);
     T2 intez = q * ((T2)EZ(0, 0, 0) + (T2)EZ(0, 1, 0) + (T2)EZ(0, 0, -1) + (T2)EZ(0, 1, -1));
-    T2 intbx = h * ((T2)BXF(0, 0, 0) + (T2)BXF(0, 0, -1));
-    T2 intby = e * ((T2)BYF(0, 0, 0) + (T2)BYF(-1, 0, 0) + (T2)BYF(-1, 1, 0) + (T2)BYF(0, 1, 0) +
-                    (T2)BYF(0, 0, -1) + (T2)BYF(-1, 0, -1) + (T2)BYF(-1, 1, -1) + (T2)BYF(0, 1, -1));
-    T2 intbz = h * ((T2)BZF(0, 0, 0) + (T2)BZF(-1, 0, 0));
-    Q2 bx_ = intbx, by_ = intby, bz_ = intbz;
-    Q2 bmag = sq3<FAST>(bx_, by_, bz_);
+    const Q2 bx_ = bb.x[1], by_ = bb.y[1], bz_ = bb.z[1], bmag = bb.m[1];
     Q2 epar = FAST ? mad_<FAST>((Q2)intez, bz_, mad_<FAST>((Q2)intey, by_, (Q2)intex * bx_))
                    : ((Q2)intex * bx_ + (Q2)intey * by_ + (Q2)intez * bz_);
     near |= bmag < (Q2)NEAR_NULL * bref || (FAST && bmag < (Q2)1e-30f);
@@ -57,12 +79,7 @@ __device__ __forceinline__ void clean_impl(const AX& EX, const AY& EY, const AZ&
     T2 intex = q * ((T2)EX(0, 0, 0) + (T2)EX(-1, 0, 0) + (T2)EX(0, 0, 1) + (T2)EX(-1, 0, 1));
     T2 intey = q * ((T2)EY(0, 0, 0) + (T2)EY(0, -1, 0) + (T2)EY(0, 0, 1) + (T2)EY(0, -1, 1));
     T2 intez = (T2)EZ(0, 0, 0);
-    T2 intbx = h * ((T2)BXF(0, 0, 0) + (T2)BXF(0, -1, 0));
-    T2 intby = h * ((T2)BYF(0, 0, 0) + (T2)BYF(-1, 0, 0));
-    T2 intbz = e * ((T2)BZF(0, 0, 0) + (T2)BZF(-1, 0, 0) + (T2)BZF(-1, -1, 0) + (T2)BZF(0, -1, 0) +
-                    (T2)BZF(0, 0, 1) + (T2)BZF(-1, 0, 1) + (T2)BZF(-1, -1, 1) + (T2)BZF(0, -1, 1));
-    Q2 bx_ = intbx, by_ = intby, bz_ = intbz;
-    Q2 bmag = sq3<FAST>(bx_, by_, bz_);
+    const Q2 bx_ = bb.x[2], by_ = bb.y[2], bz_ = bb.z[2], bmag = bb.m[2];
     Q2 epar = FAST ? mad_<FAST>((Q2)intez, bz_, mad_<FAST>((Q2)intey, by_, (Q2)intex * bx_))
                    : ((Q2)intex * bx_ + (Q2)intey * by_ + (Q2)intez * bz_);
     near |= bmag < (Q2)NEAR_NULL * bref || (FAST && bmag < (Q2)1e-30f);
@@ -78,22 +95,24 @@ __device__ __forceinline__ Q2 cell_b2(const BXa& BXF, const BYa& BYF, const BZa&
   return a * a + b * b + c * c;
 }
 
+// cleanEpar at one cell given its B interpolants `bb` and raw |B|^2 `bref`
+// (0 unless PR::null_fp64); a near-null point redoes everything in FP64.
 template <class PR, class AX, class AY, class AZ, class BXa, class BYa, class BZa>
-__device__ __forceinline__ void clean_cell(const AX& EX, const AY& EY, const AZ& EZ, const BXa& BXF,
-                                           const BYa& BYF, const BZa& BZF, int guard,
-                                           typename PR::P& ox, typename PR::P& oy,
-                                           typename PR::P& oz) {
+__device__ __forceinline__ void clean_cell_bb(const AX& EX, const AY& EY, const AZ& EZ, const BXa& BXF,
+                                              const BYa& BYF, const BZa& BZF, const BBar<typename PR::Q>& bb,
+                                              typename PR::Q bref, int guard, typename PR::P& ox,
+                                              typename PR::P& oy, typename PR::P& oz) {
   using T = typename PR::T;
   using Q = typename PR::Q;
   using P = typename PR::P;
   Q qx, qy, qz;
   bool near;
-  const Q bref = PR::null_fp64 ? cell_b2<Q>(BXF, BYF, BZF) : (Q)0;
-  clean_impl<T, Q, PR::null_fp64>(EX, EY, EZ, BXF, BYF, BZF, guard, bref, qx, qy, qz, near);
+  clean_impl<T, Q, PR::null_fp64>(EX, EY, EZ, bb, guard, bref, qx, qy, qz, near);
   if (PR::null_fp64 && near) {
     double dx, dy, dz;
     bool nn;
-    clean_impl<double, double, false>(EX, EY, EZ, BXF, BYF, BZF, guard, 0.0, dx, dy, dz, nn);
+    const BBar<double> b64 = bbar<double, double, false>(BXF, BYF, BZF);
+    clean_impl<double, double, false>(EX, EY, EZ, b64, guard, 0.0, dx, dy, dz, nn);
     ox = (P)dx;
     oy = (P)dy;
     oz = (P)dz;
@@ -104,6 +123,18 @@ __device__ __forceinline__ void clean_cell(const AX& EX, const AY& EY, const AZ&
   oz = (P)qz;
 }
 
+template <class PR, class AX, class AY, class AZ, class BXa, class BYa, class BZa>
+__device__ __forceinline__ void clean_cell(const AX& EX, const AY& EY, const AZ& EZ, const BXa& BXF,
+                                           const BYa& BYF, const BZa& BZF, int guard,
+                                           typename PR::P& ox, typename PR::P& oy,
+                                           typename PR::P& oz) {
+  using T = typename PR::T;
+  using Q = typename PR::Q;
+  const Q bref = PR::null_fp64 ? cell_b2<Q>(BXF, BYF, BZF) : (Q)0;
+  const BBar<Q> bb = bbar<T, Q, PR::null_fp64>(BXF, BYF, BZF);
+  clean_cell_bb<PR>(EX, EY, EZ, BXF, BYF, BZF, bb, bref, guard, ox, oy, oz);
+}
+
 // charge density at a node (computeRho, field_solver.F90:57-59)
 template <class T, class AX, class AY, class AZ>
 __device__ __forceinline__ T rho_node(const AX& EX, const AY& EY, const AZ& EZ, int a, int b, int d) {
@@ -112,59 +143,50 @@ __device__ __forceinline__ T rho_node(const AX& EX, const AY& EY, const AZ& EZ, 
 }
 
 // The three drift currents of rk3Push (field_solver.F90:90-157) with sums in
-// T2 and the normalisation in Q2; j[] gets Jx, Jy, Jz.
-template <class T2, class Q2, bool FAST, class AX, class AY, class AZ, class BXa, class BYa, class BZa>
-__device__ __forceinline__ void currents_impl(const AX& EX, const AY& EY, const AZ& EZ, const BXa& BXF,
-                                              const BYa& BYF, const BZa& BZF, Q2 cc, T2 r000, T2 r100,
-                                              T2 r010, T2 r001, Q2 bref, Q2* j, bool& near) {
-  const T2 h = (T2)0.5, q = (T2)0.25, e = (T2)0.125;
+// T2 and the normalisation in Q2, given the B interpolants; j[] gets Jx, Jy,
+// Jz.
+template <class T2, class Q2, bool FAST, class AX, class AY, class AZ>
+__device__ __forceinline__ void currents_impl(const AX& EX, const AY& EY, const AZ& EZ, const BBar<Q2>& bb,
+                                              Q2 cc, T2 r000, T2 r100, T2 r010, T2 r001, Q2 bref, Q2* j,
+                                              bool& near) {
+  const T2 h = (T2)0.5, q = (T2)0.25;
   near = false;
   {  // X current (:90-111)
     T2 intrho = h * (r100 + r000);
     T2 intex = (T2)EX(0, 0, 0);
     T2 intey = q * ((T2)EY(0, 0, 0) + (T2)EY(1, 0, 0) + (T2)EY(0, -1, 0) + (T2)EY(1, -1, 0));
     T2 intez = q * ((T2)EZ(0, 0, 0) + (T2)EZ(1, 0, 0) + (T2)EZ(0, 0, -1) + (T2)EZ(1, 0, -1));
-    T2 intbx = e * ((T2)BXF(0, 0, 0) + (T2)BXF(0, -1, 0) + (T2)BXF(1, -1, 0) + (T2)BXF(1, 0, 0) +
-                    (T2)BXF(0, 0, -1) + (T2)BXF(0, -1, -1) + (T2)BXF(1, -1, -1) + (T2)BXF(1, 0, -1));
-    T2 intby = h * ((T2)BYF(0, 0, 0) + (T2)BYF(0, 0, -1));
-    T2 intbz = h * ((T2)BZF(0, 0, 0) + (T2)BZF(0, -1, 0));
-    j[0] = drift_current<Q2, FAST>(cc, (Q2)intrho, (Q2)intex, (Q2)intey, (Q2)intez, (Q2)intbx, (Q2)intby,
-                             (Q2)intbz, 0, bref, near);
+    j[0] = drift_current<Q2, FAST>(cc, (Q2)intrho, (Q2)intex, (Q2)intey, (Q2)intez, bb.x[0], bb.y[0],
+                                   bb.z[0], bb.m[0], 0, bref, near);
   }
   {  // Y current (:113-134)
     T2 intrho = h * (r010 + r000);
     T2 intex = q * ((T2)EX(0, 0, 0) + (T2)EX(-1, 0, 0) + (T2)EX(0, 1, 0) + (T2)EX(-1, 1, 0));
     T2 intey = (T2)EY(0, 0, 0);
     T2 intez = q * ((T2)EZ(0, 0, 0) + (T2)EZ(0, 1, 0) + (T2)EZ(0, 0, -1) + (T2)EZ(0, 1, -1));
-    T2 intbx = h * ((T2)BXF(0, 0, 0) + (T2)BXF(0, 0, -1));
-    T2 intby = e * ((T2)BYF(0, 0, 0) + (T2)BYF(-1, 0, 0) + (T2)BYF(-1, 1, 0) + (T2)BYF(0, 1, 0) +
-                    (T2)BYF(0, 0, -1) + (T2)BYF(-1, 0, -1) + (T2)BYF(-1, 1, -1) + (T2)BYF(0, 1, -1));
-    T2 intbz = h * ((T2)BZF(0, 0, 0) + (T2)BZF(-1, 0, 0));
-    j[1] = drift_current<Q2, FAST>(cc, (Q2)intrho, (Q2)intey, (Q2)intex, (Q2)intez, (Q2)intby, (Q2)intbx,
-                             (Q2)intbz, 1, bref, near);
+    j[1] = drift_current<Q2, FAST>(cc, (Q2)intrho, (Q2)intey, (Q2)intex, (Q2)intez, bb.y[1], bb.x[1],
+                                   bb.z[1], bb.m[1], 1, bref, near);
   }
   {  // Z current (:136-157)
     T2 intrho = h * (r000 + r001);
     T2 intex = q * ((T2)EX(0, 0, 0) + (T2)EX(-1, 0, 0) + (T2)EX(0, 0, 1) + (T2)EX(-1, 0, 1));
     T2 intey = q * ((T2)EY(0, 0, 0) + (T2)EY(0, -1, 0) + (T2)EY(0, 0, 1) + (T2)EY(0, -1, 1));
     T2 intez = (T2)EZ(0, 0, 0);
-    T2 intbx = h * ((T2)BXF(0, 0, 0) + (T2)BXF(0, -1, 0));
-    T2 intby = h * ((T2)BYF(0, 0, 0) + (T2)BYF(-1, 0, 0));
-    T2 intbz = e * ((T2)BZF(0, 0, 0) + (T2)BZF(-1, 0, 0) + (T2)BZF(-1, -1, 0) + (T2)BZF(0, -1, 0) +
-                    (T2)BZF(0, 0, 1) + (T2)BZF(-1, 0, 1) + (T2)BZF(-1, -1, 1) + (T2)BZF(0, -1, 1));
-    j[2] = drift_current<Q2, FAST>(cc, (Q2)intrho, (Q2)intez, (Q2)intex, (Q2)intey, (Q2)intbz, (Q2)intbx,
-                             (Q2)intby, 2, bref, near);
+    j[2] = drift_current<Q2, FAST>(cc, (Q2)intrho, (Q2)intez, (Q2)intex, (Q2)intey, bb.z[2], bb.x[2],
+                                   bb.y[2], bb.m[2], 2, bref, near);
   }
 }
 
-// rk3Push at one cell (field_solver.F90:81-162): d[0..2] = dE, d[3..5] = dB.
+// rk3Push at one cell (field_solver.F90:81-162) given the B interpolants
+// `bb` and raw |B|^2 `bref` of the cell: d[0..2] = dE, d[3..5] = dB.
 // RHO(a, b, d) gives the charge density at node (i+a, j+b, k+d); t0 is the
 // precomputed B0 curl bracket (zero for a uniform background).
 template <class PR, class AX, class AY, class AZ, class BXa, class BYa, class BZa, class RF>
-__device__ __forceinline__ void rhs_cell(const AX& EX, const AY& EY, const AZ& EZ, const BXa& BXF,
-                                         const BYa& BYF, const BZa& BZF, const RF& RHO,
-                                         typename PR::P cc, typename PR::T tx, typename PR::T ty,
-                                         typename PR::T tz, typename PR::P* d) {
+__device__ __forceinline__ void rhs_cell_bb(const AX& EX, const AY& EY, const AZ& EZ, const BXa& BXF,
+                                            const BYa& BYF, const BZa& BZF, const RF& RHO,
+                                            const BBar<typename PR::Q>& bb, typename PR::Q bref,
+                                            typename PR::P cc, typename PR::T tx, typename PR::T ty,
+                                            typename PR::T tz, typename PR::P* d) {
   using T = typename PR::T;
   using Q = typename PR::Q;
   using P = typename PR::P;
@@ -180,14 +202,13 @@ __device__ __forceinline__ void rhs_cell(const AX& EX, const AY& EY, const AZ& E
   const T r001 = RHO(0, 0, 1);
   Q j[3];
   bool near;
-  const Q bref = PR::null_fp64 ? cell_b2<Q>(BXF, BYF, BZF) : (Q)0;
-  currents_impl<T, Q, PR::null_fp64>(EX, EY, EZ, BXF, BYF, BZF, (Q)cc, r000, r100, r010, r001, bref, j,
-                                     near);
+  currents_impl<T, Q, PR::null_fp64>(EX, EY, EZ, bb, (Q)cc, r000, r100, r010, r001, bref, j, near);
   if (PR::null_fp64 && near) {
     double jd[3];
     bool nn;
-    currents_impl<double, double, false>(EX, EY, EZ, BXF, BYF, BZF, (double)cc, (double)r000, (double)r100,
-                                  (double)r010, (double)r001, 0.0, jd, nn);
+    const BBar<double> b64 = bbar<double, double, false>(BXF, BYF, BZF);
+    currents_impl<double, double, false>(EX, EY, EZ, b64, (double)cc, (double)r000, (double)r100,
+                                         (double)r010, (double)r001, 0.0, jd, nn);
     dex = dex - (P)jd[0];
     dey = dey - (P)jd[1];
     dez = dez - (P)jd[2];
@@ -200,6 +221,18 @@ __device__ __forceinline__ void rhs_cell(const AX& EX, const AY& EY, const AZ& E
   d[3] = dbx; d[4] = dby; d[5] = dbz;
 }
 
+template <class PR, class AX, class AY, class AZ, class BXa, class BYa, class BZa, class RF>
+__device__ __forceinline__ void rhs_cell(const AX& EX, const AY& EY, const AZ& EZ, const BXa& BXF,
+                                         const BYa& BYF, const BZa& BZF, const RF& RHO,
+                                         typename PR::P cc, typename PR::T tx, typename PR::T ty,
+                                         typename PR::T tz, typename PR::P* d) {
+  using T = typename PR::T;
+  using Q = typename PR::Q;
+  const Q bref = PR::null_fp64 ? cell_b2<Q>(BXF, BYF, BZF) : (Q)0;
+  const BBar<Q> bb = bbar<T, Q, PR::null_fp64>(BXF, BYF, BZF);
+  rhs_cell_bb<PR>(EX, EY, EZ, BXF, BYF, BZF, RHO, bb, bref, cc, tx, ty, tz, d);
+}
+
 // rk3Update (field_solver.F90:187-193 and the absorbing layer :200-265) at
 // one interior cell: v holds X on entry (the c2 term), u holds U^n; on exit
 // v holds the new values in working precision, ready to round to S.
@@ -210,9 +243,16 @@ __device__ __forceinline__ void update_cell(typename PR::P* v, const typename PR
                                             const Absorb<typename PR::P>& ab, int i, int j, int k) {
   using S = typename PR::S;
   using P = typename PR::P;
-  for (int m = 0; m < 6; ++m)
-    v[m] = PR::null_fp64 ? mad_<true>(c3, d[m], mad_<true>(c2, v[m], c1 * u[m]))
-                         : c1 * u[m] + c2 * v[m] + c3 * d[m];
+  if (PR::null_fp64 && c2 == c3 && c1 + c2 == (P)1) {
+    // the SSP-RK3 stages all have c1 = 1 - c and c2 = c3 = c: U^n + c ((X -
+    // U^n) + dU) keeps the state's scale exact, where rounded 1/3 and 2/3
+    // would drift it by their rounding every step
+    for (int m = 0; m < 6; ++m) v[m] = mad_<true>(c2, (v[m] - u[m]) + d[m], u[m]);
+  } else {
+    for (int m = 0; m < 6; ++m)
+      v[m] = PR::null_fp64 ? mad_<true>(c3, d[m], mad_<true>(c2, v[m], c1 * u[m]))
+                           : c1 * u[m] + c2 * v[m] + c3 * d[m];
+  }
   if (ab.on) {
     // the reference stores first and damps the stored value
     for (int m = 0; m < 6; ++m) v[m] = (P)(S)v[m];

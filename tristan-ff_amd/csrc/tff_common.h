@@ -54,11 +54,14 @@ struct PrecF32 {
   using P = float;
   static constexpr bool null_fp64 = false;
 };
+#ifndef TFF_BULK_P
+#define TFF_BULK_P float
+#endif
 struct PrecF32Bulk {
   using S = float;
   using T = float;
   using Q = float;
-  using P = double;
+  using P = TFF_BULK_P;
   static constexpr bool null_fp64 = true;
 };
 
@@ -104,27 +107,27 @@ __device__ __forceinline__ Q sq3(Q x, Q y, Q z) {
 // the current's own axis when |E| > |B|, then
 // j = CC / (|B|^2 + TINY) * (rho * (Ea*Bb - Ba*Eb)).
 // `self` is the E component along the current's axis (never rescaled).
+// `bmag` is |B|^2 at the point, summed in the order x, y, z.
 template <class P, bool FAST = false>
-__device__ __forceinline__ P drift_current(P cc, P intrho, P self, P ea, P eb, P bs, P ba, P bb,
+__device__ __forceinline__ P drift_current(P cc, P intrho, P self, P ea, P eb, P bs, P ba, P bb, P bmag,
                                            int comp, P bref, bool& near) {
   // comp 0 (x): self=Ex, ea=Ey, eb=Ez, bs=Bx, ba=By, bb=Bz
-  P bmag, emag;
-  if (comp == 0) {
-    bmag = sq3<FAST>(bs, ba, bb);
+  // comp 1 (y): self=Ey, ea=Ex, eb=Ez, bs=By, ba=Bx, bb=Bz
+  // comp 2 (z): self=Ez, ea=Ex, eb=Ey, bs=Bz, ba=Bx, bb=By
+  // (sums of squares in the order x, y, z)
+  P emag;
+  if (comp == 0)
     emag = sq3<FAST>(self, ea, eb);
-  } else if (comp == 1) {
-    // ordering of the sums follows x,y,z: here self=Ey, ea=Ex, eb=Ez,
-    // bs=By, ba=Bx, bb=Bz
-    bmag = sq3<FAST>(ba, bs, bb);
+  else if (comp == 1)
     emag = sq3<FAST>(ea, self, eb);
-  } else {
-    // self=Ez, ea=Ex, eb=Ey, bs=Bz, ba=Bx, bb=By
-    bmag = sq3<FAST>(ba, bb, bs);
+  else
     emag = sq3<FAST>(ea, eb, self);
-  }
   near |= bmag < (P)NEAR_NULL * bref;
   if (emag > bmag) {
-    P corr = sqrt(div_<FAST>(bmag, emag));
+    // IEEE division even on the fast path: this branch is rare, and the
+    // hardware reciprocal of an |E|^2 below 2^-128 overflows to +inf, which
+    // turns bmag = 0 (a zeroed region) into 0 * inf = NaN
+    P corr = sqrt(bmag / emag);
     ea = ea * corr;
     eb = eb * corr;
   }

@@ -13,14 +13,20 @@
 //                needs its own neighbours),
 //   cleaned E    with a 1-cell halo (the stencil of rk3Push),
 //   rho          on the (TX+1) x (TY+1) nodes the currents need.
-// Each ring keeps only the planes its readers still need: ex, ey raw and
-// cleaned 2, ez raw and cleaned 3, B 3 (bx, by of plane k+2 are stored
-// only after the stencil of plane k has read plane k-1), rho 2.
+// Each ring keeps the planes its readers still need while the next raw
+// plane is stored: ex, ey raw and cleaned 2, ez raw and cleaned 3, bz 3,
+// bx and by 4 (the curl of plane k reads plane k-1), rho 2.
 //
-// Per iteration (output plane k): load raw plane k+2, clean plane k+1 on
-// the halo, rho on plane k+1, then rk3Push/rk3Update on plane k.  The
-// per-cell arithmetic is tff_cell.h's, so results equal the unfused
-// kernels bit for bit.
+// Per iteration (output plane k): store raw plane k+2 (loaded into
+// registers during the previous iteration) and issue the loads of plane k+3
+// and of U^n at plane k, so that their latency overlaps this iteration's
+// work; clean plane k+1 on the halo, rho on plane k+1, then rk3Push /
+// rk3Update on plane k.  The per-cell arithmetic is tff_cell.h's, so
+// results equal the unfused kernels bit for bit.
+//
+// Addressing: every thread precomputes the in-plane (x-y) element offsets of
+// the cells it loads or writes once per tile; per plane only a uniform base
+// pointer moves, so global accesses take a scalar base plus a 32-bit offset.
 //
 // Ghost cells: with every axis periodic (`wrap`), halo reads wrap around the
 // box and no ghost cell is read or written.  Otherwise ghosts are stored:
@@ -41,7 +47,7 @@ struct Tile {
   static constexpr int L2PT = (N2 + NT - 1) / NT;
   template <class S, class T>
   static constexpr size_t lds_bytes() {
-    return sizeof(S) * (size_t)(16 * N2 + 7 * N1) + sizeof(T) * (size_t)(2 * NN);
+    return sizeof(S) * (size_t)(18 * N2 + 7 * N1) + sizeof(T) * (size_t)(2 * NN);
   }
 };
 
@@ -72,8 +78,36 @@ __device__ __forceinline__ int wrap_index(int v, int n) {
   return v;
 }
 
+// element at a uniform base plus a 32-bit byte offset (the form global
+// memory instructions take as scalar base + vector offset)
+template <class S>
+__device__ __forceinline__ S ld_at(const S* base, uint32_t off) {
+  return *(const S*)((const char*)base + off);
+}
+template <class S>
+__device__ __forceinline__ void st_at(S* base, uint32_t off, S v) {
+  *(S*)((char*)base + off) = v;
+}
+
+// LDS accessor of one field around one cell: p[d+1] points at neighbour
+// (-1, -1) of the cell in plane slot d = -1, 0, +1, so every stencil offset
+// (a, b) in -1..1 is a small non-negative constant from one of three bases.
+template <class S, int W>
+struct LdsCell {
+  const S* p[3];
+  __device__ __forceinline__ S operator()(int a, int b, int d) const { return p[d + 1][(a + 1) + (b + 1) * W]; }
+};
+template <class S, int W, class SLOT>
+__device__ __forceinline__ LdsCell<S, W> lds_cell(const S* ring, int plane_size, int q, int cell, SLOT slot) {
+  LdsCell<S, W> r;
+#pragma unroll
+  for (int d = -1; d <= 1; ++d) r.p[d + 1] = ring + slot(q + d) * plane_size + cell - W - 1;
+  return r;
+}
+
 __device__ __forceinline__ int s2(int p) { return p & 1; }
 __device__ __forceinline__ int s3(int p) { return (p + 6) % 3; }
+__device__ __forceinline__ int s4(int p) { return p & 3; }
 
 // One tile (tile_id in x-fastest, then y, then z-chunk order) of a stage.
 template <class PR, class TL>
@@ -81,15 +115,16 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
                                            unsigned char* smem_raw) {
   using S = typename PR::S;
   using T = typename PR::T;
+  using Q = typename PR::Q;
   using P = typename PR::P;
   constexpr int TX = TL::TX, TY = TL::TY, W2 = TL::W2, N2 = TL::N2, W1 = TL::W1, N1 = TL::N1;
   constexpr int WN = TL::WN, NN = TL::NN, NT = TL::NT, L2PT = TL::L2PT;
   S* rex = (S*)smem_raw;         // [2][N2]
   S* rey = rex + 2 * N2;         // [2][N2]
   S* rez = rey + 2 * N2;         // [3][N2]
-  S* rbx = rez + 3 * N2;         // [3][N2]
-  S* rby = rbx + 3 * N2;         // [3][N2]
-  S* rbz = rby + 3 * N2;         // [3][N2]
+  S* rbx = rez + 3 * N2;         // [4][N2]
+  S* rby = rbx + 4 * N2;         // [4][N2]
+  S* rbz = rby + 4 * N2;         // [3][N2]
   S* cex = rbz + 3 * N2;         // [2][N1]
   S* cey = cex + 2 * N1;         // [2][N1]
   S* cez = cey + 2 * N1;         // [3][N1]
@@ -104,41 +139,52 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
   const int i0 = bxi * TX, j0 = byi * TY;
   const int k0 = bzi * A.kchunk;
   const int k1 = min(k0 + A.kchunk, g.nz);
-  // tiles whose 2-cell halo stays inside the interior need no wrapping
-  const bool inner_xy = i0 >= NG && i0 + TX + NG <= g.nx && j0 >= NG && j0 + TY + NG <= g.ny;
 
-  auto addr = [&](int gi, int gj, int gk) -> int64_t {
+  // x-y byte offsets (from the plane origin) of this thread's halo cells
+  // of the raw planes: wrapped on periodic boxes, clamped to the stored
+  // ghosts otherwise
+  uint32_t hxy[L2PT];
+#pragma unroll
+  for (int m = 0; m < L2PT; ++m) {
+    const int c = min(tid + m * NT, N2 - 1);
+    int gi = i0 - 2 + c % W2, gj = j0 - 2 + c / W2;
     if (A.wrap) {
-      if (!inner_xy) {
-        gi = wrap_index(gi, g.nx);
-        gj = wrap_index(gj, g.ny);
-      }
-      gk = wrap_index(gk, g.nz);
+      gi = wrap_index(gi, g.nx);
+      gj = wrap_index(gj, g.ny);
     } else {
       gi = min(gi, g.nx + NG - 1);
       gj = min(gj, g.ny + NG - 1);
     }
-    return g.at(gi, gj, gk);
+    hxy[m] = (uint32_t)((g.org + gi + (int64_t)gj * g.px) * (int64_t)sizeof(S));
+  }
+  // uniform pointer to plane p of a buffer's component 0
+  auto plane_of = [&](const S* base, int p) -> const S* {
+    const int gk = A.wrap ? wrap_index(p, g.nz) : p;
+    return base + (int64_t)gk * g.plane;
   };
 
   S reg[L2PT][6];
   auto load_plane = [&](int p) {
+    const S* e0 = plane_of(A.ye, p);
+    const S* b0 = plane_of(A.yb, p);
+    const S* e1 = e0 + cs;
+    const S* e2 = e0 + 2 * cs;
+    const S* b1 = b0 + cs;
+    const S* b2 = b0 + 2 * cs;
 #pragma unroll
     for (int m = 0; m < L2PT; ++m) {
-      const int c = tid + m * NT;
-      if (c < N2) {
-        const int lx = c % W2, ly = c / W2;
-        const int64_t a = addr(i0 - 2 + lx, j0 - 2 + ly, p);
-        reg[m][0] = A.ye[a];
-        reg[m][1] = A.ye[a + cs];
-        reg[m][2] = A.ye[a + 2 * cs];
-        reg[m][3] = A.yb[a];
-        reg[m][4] = A.yb[a + cs];
-        reg[m][5] = A.yb[a + 2 * cs];
+      if (tid + m * NT < N2) {
+        const uint32_t o = hxy[m];
+        reg[m][0] = ld_at(e0, o);
+        reg[m][1] = ld_at(e1, o);
+        reg[m][2] = ld_at(e2, o);
+        reg[m][3] = ld_at(b0, o);
+        reg[m][4] = ld_at(b1, o);
+        reg[m][5] = ld_at(b2, o);
       }
     }
   };
-  auto store_plane = [&](int p, bool with_bxy) {
+  auto store_plane = [&](int p) {
 #pragma unroll
     for (int m = 0; m < L2PT; ++m) {
       const int c = tid + m * NT;
@@ -146,58 +192,90 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
         rex[s2(p) * N2 + c] = reg[m][0];
         rey[s2(p) * N2 + c] = reg[m][1];
         rez[s3(p) * N2 + c] = reg[m][2];
+        rbx[s4(p) * N2 + c] = reg[m][3];
+        rby[s4(p) * N2 + c] = reg[m][4];
         rbz[s3(p) * N2 + c] = reg[m][5];
-        if (with_bxy) {
-          rbx[s3(p) * N2 + c] = reg[m][3];
-          rby[s3(p) * N2 + c] = reg[m][4];
-        }
-      }
-    }
-  };
-  auto store_bxy = [&](int p) {
-#pragma unroll
-    for (int m = 0; m < L2PT; ++m) {
-      const int c = tid + m * NT;
-      if (c < N2) {
-        rbx[s3(p) * N2 + c] = reg[m][3];
-        rby[s3(p) * N2 + c] = reg[m][4];
       }
     }
   };
 
-  // cleanEpar of plane q on the 1-cell halo
+  // The B interpolants and raw |B|^2 of a thread's own cell serve both the
+  // clean of that cell and, one plane later, its drift currents (they are
+  // the same expressions of the same stage-input B).  FP32 policies keep
+  // them in registers between the two; FP64 recomputes them (register
+  // budget).
+  constexpr bool REUSE = sizeof(Q) == 4;
+  constexpr int RING = 2 * W1 + 2 * TY;  // halo cells of the cleaned region
+  BBar<Q> nbb{}, cbb{};
+  Q nbref = (Q)0, cbref = (Q)0;
+
+  // cleanEpar of cell (mx, my) of the cleaned region of plane q
+  auto clean_at = [&](int q, int mx, int my, bool own) {
+    const int c = my * W1 + mx;
+    const int lc = (my + 1) * W2 + (mx + 1);  // same cell in the raw plane
+    const int gi = i0 - 1 + mx, gj = j0 - 1 + my;
+    const auto lex = lds_cell<S, W2>(rex, N2, q, lc, s2), ley = lds_cell<S, W2>(rey, N2, q, lc, s2);
+    const auto lez = lds_cell<S, W2>(rez, N2, q, lc, s3), lbx = lds_cell<S, W2>(rbx, N2, q, lc, s4);
+    const auto lby = lds_cell<S, W2>(rby, N2, q, lc, s4), lbz = lds_cell<S, W2>(rbz, N2, q, lc, s3);
+    auto EX = [&](int a, int b, int d) { return (T)lex(a, b, d); };
+    auto EY = [&](int a, int b, int d) { return (T)ley(a, b, d); };
+    auto EZ = [&](int a, int b, int d) { return (T)lez(a, b, d); };
+    auto BXF = [&](int a, int b, int d) { return (T)lbx(a, b, d); };
+    auto BYF = [&](int a, int b, int d) { return (T)lby(a, b, d); };
+    auto BZF = [&](int a, int b, int d) { return (T)lbz(a, b, d); };
+    S ox, oy, oz;
+    const bool ghost = !A.wrap && (gi < 0 || gi >= g.nx || gj < 0 || gj >= g.ny || q < 0 || q >= g.nz);
+    const bool project = A.clean_in && !ghost;
+    BBar<Q> bb{};
+    Q bref = (Q)0;
+    if (project || (REUSE && own)) {
+      bb = bbar<T, Q, PR::null_fp64>(BXF, BYF, BZF);
+      if (PR::null_fp64) bref = cell_b2<Q>(BXF, BYF, BZF);
+    }
+    if (!A.clean_in) {
+      ox = lex(0, 0, 0);
+      oy = ley(0, 0, 0);
+      oz = lez(0, 0, 0);
+    } else if (ghost) {
+      const int64_t a = g.at(min(gi, g.nx + NG - 1), min(gj, g.ny + NG - 1), q);
+      ox = A.ge[a];
+      oy = A.ge[a + cs];
+      oz = A.ge[a + 2 * cs];
+    } else {
+      P px, py, pz;
+      clean_cell_bb<PR>(EX, EY, EZ, BXF, BYF, BZF, bb, bref, A.guard, px, py, pz);
+      ox = (S)px;
+      oy = (S)py;
+      oz = (S)pz;
+    }
+    cex[s2(q) * N1 + c] = ox;
+    cey[s2(q) * N1 + c] = oy;
+    cez[s3(q) * N1 + c] = oz;
+    if (REUSE && own) {
+      nbb = bb;
+      nbref = bref;
+    }
+  };
+  // cleanEpar of plane q on the tile and its 1-cell halo: each thread its
+  // own cell, then the first RING threads the halo ring
   auto clean_plane = [&](int q) {
-    for (int c = tid; c < N1; c += NT) {
-      const int mx = c % W1, my = c / W1;
-      const int lc = (my + 1) * W2 + (mx + 1);  // same cell in the raw plane
-      const int gi = i0 - 1 + mx, gj = j0 - 1 + my;
-      S ox, oy, oz;
-      const bool ghost = !A.wrap && (gi < 0 || gi >= g.nx || gj < 0 || gj >= g.ny || q < 0 || q >= g.nz);
-      if (!A.clean_in) {
-        ox = rex[s2(q) * N2 + lc];
-        oy = rey[s2(q) * N2 + lc];
-        oz = rez[s3(q) * N2 + lc];
-      } else if (ghost) {
-        const int64_t a = g.at(min(gi, g.nx + NG - 1), min(gj, g.ny + NG - 1), q);
-        ox = A.ge[a];
-        oy = A.ge[a + cs];
-        oz = A.ge[a + 2 * cs];
+    clean_at(q, tid % TX + 1, tid / TX + 1, true);
+    for (int r = tid; r < RING; r += NT) {
+      int mx, my;
+      if (r < W1) {
+        mx = r;
+        my = 0;
+      } else if (r < 2 * W1) {
+        mx = r - W1;
+        my = TY + 1;
+      } else if (r < 2 * W1 + TY) {
+        mx = 0;
+        my = r - 2 * W1 + 1;
       } else {
-        auto EX = [&](int a, int b, int d) { return (T)rex[s2(q + d) * N2 + lc + a + b * W2]; };
-        auto EY = [&](int a, int b, int d) { return (T)rey[s2(q + d) * N2 + lc + a + b * W2]; };
-        auto EZ = [&](int a, int b, int d) { return (T)rez[s3(q + d) * N2 + lc + a + b * W2]; };
-        auto BXF = [&](int a, int b, int d) { return (T)rbx[s3(q + d) * N2 + lc + a + b * W2]; };
-        auto BYF = [&](int a, int b, int d) { return (T)rby[s3(q + d) * N2 + lc + a + b * W2]; };
-        auto BZF = [&](int a, int b, int d) { return (T)rbz[s3(q + d) * N2 + lc + a + b * W2]; };
-        P px, py, pz;
-        clean_cell<PR>(EX, EY, EZ, BXF, BYF, BZF, A.guard, px, py, pz);
-        ox = (S)px;
-        oy = (S)py;
-        oz = (S)pz;
+        mx = W1 - 1;
+        my = r - 2 * W1 - TY + 1;
       }
-      cex[s2(q) * N1 + c] = ox;
-      cey[s2(q) * N1 + c] = oy;
-      cez[s3(q) * N1 + c] = oz;
+      clean_at(q, mx, my, false);
     }
   };
 
@@ -206,29 +284,34 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
     for (int c = tid; c < NN; c += NT) {
       const int nx_ = c % WN, ny_ = c / WN;
       const int lc = (ny_ + 1) * W1 + (nx_ + 1);
-      auto EX = [&](int a, int b, int d) { return (T)cex[s2(q + d) * N1 + lc + a + b * W1]; };
-      auto EY = [&](int a, int b, int d) { return (T)cey[s2(q + d) * N1 + lc + a + b * W1]; };
-      auto EZ = [&](int a, int b, int d) { return (T)cez[s3(q + d) * N1 + lc + a + b * W1]; };
+      const auto lex = lds_cell<S, W1>(cex, N1, q, lc, s2), ley = lds_cell<S, W1>(cey, N1, q, lc, s2);
+      const auto lez = lds_cell<S, W1>(cez, N1, q, lc, s3);
+      auto EX = [&](int a, int b, int d) { return (T)lex(a, b, d); };
+      auto EY = [&](int a, int b, int d) { return (T)ley(a, b, d); };
+      auto EZ = [&](int a, int b, int d) { return (T)lez(a, b, d); };
       rho[s2(q) * NN + c] = rho_node<T>(EX, EY, EZ, 0, 0, 0);
     }
   };
 
   // prologue: planes k0-2 .. k0+1 raw, k0-1 and k0 cleaned, rho at k0
   load_plane(k0 - 2);
-  store_plane(k0 - 2, true);
+  store_plane(k0 - 2);
   load_plane(k0 - 1);
-  store_plane(k0 - 1, true);
+  store_plane(k0 - 1);
   load_plane(k0);
-  store_plane(k0, true);
+  store_plane(k0);
   __syncthreads();
   clean_plane(k0 - 1);
   __syncthreads();
   load_plane(k0 + 1);
-  store_plane(k0 + 1, true);
+  store_plane(k0 + 1);
   __syncthreads();
   clean_plane(k0);
   __syncthreads();
   rho_plane(k0);
+  cbb = nbb;
+  cbref = nbref;
+  if (k0 < k1) load_plane(k0 + 2);
 
   const int tx = tid % TX, ty = tid / TX;
   const int gi = i0 + tx, gj = j0 + ty;
@@ -236,58 +319,97 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
   const int lc1 = (ty + 1) * W1 + (tx + 1);
   const int lc2 = (ty + 2) * W2 + (tx + 2);
   const int lcn = ty * WN + tx;
+  // x-y byte offset of the thread's own cell (clamped for inactive threads)
+  const uint32_t oxy =
+      (uint32_t)((g.org + min(gi, g.nx - 1) + (int64_t)min(gj, g.ny - 1) * g.px) * (int64_t)sizeof(S));
 
   for (int k = k0; k < k1; ++k) {
-    load_plane(k + 2);
-    store_plane(k + 2, false);
+    // raw plane k+2 arrived during the previous iteration
+    store_plane(k + 2);
+    const int64_t pk = (int64_t)k * g.plane;
+    // U^n and the B0 curl of plane k, then raw plane k+3, in flight while
+    // this iteration cleans and computes (FP64 loads U^n where it is used:
+    // two more FP64 sextets in flight would cost a wave per SIMD)
+    constexpr bool EARLY = sizeof(S) == 4;
+    S un[6];
+    T tt[3] = {(T)0, (T)0, (T)0};
+    auto load_un = [&]() {
+      if (!A.stage1) {
+        const S* ue = A.ue + pk;
+        const S* ub = A.ub + pk;
+        un[0] = ld_at(ue, oxy);
+        un[1] = ld_at(ue + cs, oxy);
+        un[2] = ld_at(ue + 2 * cs, oxy);
+        un[3] = ld_at(ub, oxy);
+        un[4] = ld_at(ub + cs, oxy);
+        un[5] = ld_at(ub + 2 * cs, oxy);
+      }
+      if (A.has_t0) {
+        const S* t0 = A.t0 + pk;
+        tt[0] = (T)ld_at(t0, oxy);
+        tt[1] = (T)ld_at(t0 + cs, oxy);
+        tt[2] = (T)ld_at(t0 + 2 * cs, oxy);
+      }
+    };
+    if (EARLY && active) load_un();
+    if (k + 1 < k1) load_plane(k + 3);
     __syncthreads();
     clean_plane(k + 1);
     __syncthreads();
     rho_plane(k + 1);
     __syncthreads();
     if (active) {
-      auto EX = [&](int a, int b, int d) { return (T)cex[s2(k + d) * N1 + lc1 + a + b * W1]; };
-      auto EY = [&](int a, int b, int d) { return (T)cey[s2(k + d) * N1 + lc1 + a + b * W1]; };
-      auto EZ = [&](int a, int b, int d) { return (T)cez[s3(k + d) * N1 + lc1 + a + b * W1]; };
-      auto BXF = [&](int a, int b, int d) { return (T)rbx[s3(k + d) * N2 + lc2 + a + b * W2]; };
-      auto BYF = [&](int a, int b, int d) { return (T)rby[s3(k + d) * N2 + lc2 + a + b * W2]; };
-      auto BZF = [&](int a, int b, int d) { return (T)rbz[s3(k + d) * N2 + lc2 + a + b * W2]; };
-      auto RHO = [&](int a, int b, int d) { return rho[s2(k + d) * NN + lcn + a + b * WN]; };
-      const int64_t c = g.at(gi, gj, k);
-      const T t0x = A.has_t0 ? (T)A.t0[c] : (T)0;
-      const T t0y = A.has_t0 ? (T)A.t0[c + cs] : (T)0;
-      const T t0z = A.has_t0 ? (T)A.t0[c + 2 * cs] : (T)0;
+      if (!EARLY) load_un();
+      const auto lex = lds_cell<S, W1>(cex, N1, k, lc1, s2), ley = lds_cell<S, W1>(cey, N1, k, lc1, s2);
+      const auto lez = lds_cell<S, W1>(cez, N1, k, lc1, s3);
+      const auto lbx = lds_cell<S, W2>(rbx, N2, k, lc2, s4), lby = lds_cell<S, W2>(rby, N2, k, lc2, s4);
+      const auto lbz = lds_cell<S, W2>(rbz, N2, k, lc2, s3);
+      const auto lrho = lds_cell<T, WN>(rho, NN, k, lcn, s2);
+      auto EX = [&](int a, int b, int d) { return (T)lex(a, b, d); };
+      auto EY = [&](int a, int b, int d) { return (T)ley(a, b, d); };
+      auto EZ = [&](int a, int b, int d) { return (T)lez(a, b, d); };
+      auto BXF = [&](int a, int b, int d) { return (T)lbx(a, b, d); };
+      auto BYF = [&](int a, int b, int d) { return (T)lby(a, b, d); };
+      auto BZF = [&](int a, int b, int d) { return (T)lbz(a, b, d); };
+      auto RHO = [&](int a, int b, int d) { return lrho(a, b, d); };
       P d[6];
-      rhs_cell<PR>(EX, EY, EZ, BXF, BYF, BZF, RHO, A.cc, t0x, t0y, t0z, d);
-      P v[6] = {(P)cex[s2(k) * N1 + lc1], (P)cey[s2(k) * N1 + lc1], (P)cez[s3(k) * N1 + lc1],
-                (P)rbx[s3(k) * N2 + lc2], (P)rby[s3(k) * N2 + lc2], (P)rbz[s3(k) * N2 + lc2]};
+      if constexpr (REUSE)
+        rhs_cell_bb<PR>(EX, EY, EZ, BXF, BYF, BZF, RHO, cbb, cbref, A.cc, tt[0], tt[1], tt[2], d);
+      else
+        rhs_cell<PR>(EX, EY, EZ, BXF, BYF, BZF, RHO, A.cc, tt[0], tt[1], tt[2], d);
+      P v[6] = {(P)lex(0, 0, 0), (P)ley(0, 0, 0), (P)lez(0, 0, 0),
+                (P)lbx(0, 0, 0), (P)lby(0, 0, 0), (P)lbz(0, 0, 0)};
       P u[6];
       if (A.stage1) {
         for (int m = 0; m < 6; ++m) u[m] = v[m];
         if (A.u0e) {
-          A.u0e[c] = cex[s2(k) * N1 + lc1];
-          A.u0e[c + cs] = cey[s2(k) * N1 + lc1];
-          A.u0e[c + 2 * cs] = cez[s3(k) * N1 + lc1];
+          S* u0 = A.u0e + pk;
+          st_at(u0, oxy, lex(0, 0, 0));
+          st_at(u0 + cs, oxy, ley(0, 0, 0));
+          st_at(u0 + 2 * cs, oxy, lez(0, 0, 0));
         }
       } else {
-        u[0] = (P)A.ue[c]; u[1] = (P)A.ue[c + cs]; u[2] = (P)A.ue[c + 2 * cs];
-        u[3] = (P)A.ub[c]; u[4] = (P)A.ub[c + cs]; u[5] = (P)A.ub[c + 2 * cs];
+        for (int m = 0; m < 6; ++m) u[m] = (P)un[m];
       }
       update_cell<PR>(v, u, d, A.c1, A.c2, A.c3, A.ab, gi, gj, k);
-      A.oe[c] = (S)v[0];
-      A.oe[c + cs] = (S)v[1];
-      A.oe[c + 2 * cs] = (S)v[2];
-      A.ob[c] = (S)v[3];
-      A.ob[c + cs] = (S)v[4];
-      A.ob[c + 2 * cs] = (S)v[5];
+      S* oe = A.oe + pk;
+      S* ob = A.ob + pk;
+      st_at(oe, oxy, (S)v[0]);
+      st_at(oe + cs, oxy, (S)v[1]);
+      st_at(oe + 2 * cs, oxy, (S)v[2]);
+      st_at(ob, oxy, (S)v[3]);
+      st_at(ob + cs, oxy, (S)v[4]);
+      st_at(ob + 2 * cs, oxy, (S)v[5]);
     }
-    __syncthreads();
-    store_bxy(k + 2);
+    cbb = nbb;
+    cbref = nbref;
+    __syncthreads();  // plane k's rings are free for plane k+3 / k+2
   }
 }
 
 template <class PR, class TL>
-__global__ void __launch_bounds__(TL::NT) k_stage(StageArgs<typename PR::S, typename PR::P> A) {
+__global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(sizeof(typename PR::S) == 4 ? 3 : 2)))
+    k_stage(StageArgs<typename PR::S, typename PR::P> A) {
   extern __shared__ unsigned char smem_raw[];
   // Workgroups are dealt to the 8 XCDs round-robin; give each XCD one
   // contiguous run of tiles (x fastest, then y, then z-chunk) so that the

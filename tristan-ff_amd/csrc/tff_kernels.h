@@ -213,6 +213,36 @@ __device__ __forceinline__ P loop_bz(P bnorm, P rx, P ry1, P ry2, P rz) {
   return bnorm * (rz / (rr1 * rr1 * rr1) - rz / (rr2 * rr2 * rr2));
 }
 
+// The footpoint driver at (i, j, k): Ex, Ey and the potential-field Bz.
+template <class P>
+__device__ __forceinline__ void loop_driver(const double* p, P cc, int step, int i, int j, int k, P& ex, P& ey,
+                                            P& bz) {
+  const P xc = (P)p[0], yc1 = (P)p[1], yc2 = (P)p[2], zc = (P)p[3], bnorm = (P)p[4];
+  const P r0 = (P)p[5], beta = (P)p[6], t0 = (P)p[7];
+  const P xg = (P)i, yg = (P)j, zg = (P)k, hf = (P)0.5, time = (P)step;
+  P rx, ry1, ry2, rz, rr1, bzl, omega, ee0;
+  rx = xg + hf - xc; ry1 = yg - yc1; ry2 = yg - yc2; rz = zg - zc;
+  rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
+  bzl = loop_bz(bnorm, rx, ry1, ry2, rz);
+  omega = exp(-rr1 / r0) * beta * cc / r0;
+  ee0 = -omega * bzl * tanh(time / t0);
+  ex = rx * ee0;
+  rx = xg - xc; ry1 = yg + hf - yc1; ry2 = yg + hf - yc2; rz = zg - zc;
+  rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
+  bzl = loop_bz(bnorm, rx, ry1, ry2, rz);
+  omega = exp(-rr1 / r0) * beta * cc / r0;
+  ee0 = -omega * bzl * tanh(time / t0);
+  ey = ry1 * ee0;
+  bz = loop_bz(bnorm, xg + hf - xc, yg + hf - yc1, yg + hf - yc2, zg - zc);
+}
+
+// The conducting top of the driven loop: Bz of the potential field.
+template <class P>
+__device__ __forceinline__ P loop_top_bz(const double* p, int i, int j, int k) {
+  const P xc = (P)p[0], yc1 = (P)p[1], yc2 = (P)p[2], zc = (P)p[3], bnorm = (P)p[4], hf = (P)0.5;
+  return loop_bz(bnorm, (P)i + hf - xc, (P)j + hf - yc1, (P)j + hf - yc2, (P)k - zc);
+}
+
 template <class PR>
 __global__ void k_bc_loop(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, const double* p,
                           typename PR::P cc, const int* step_ptr, int step_val) {
@@ -228,28 +258,13 @@ __global__ void k_bc_loop(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, co
   const int mode = (int)p[8];
   if (k == 0 && mode != 2) return;
   const int64_t c = g.at(i, j, k);
-  const P xc = (P)p[0], yc1 = (P)p[1], yc2 = (P)p[2], zc = (P)p[3], bnorm = (P)p[4];
-  const P xg = (P)i, yg = (P)j, zg = (P)k, hf = (P)0.5;
   if (k <= 0) {
     if (mode != 0) {
-      const int step = step_ptr ? *step_ptr : step_val;
-      const P r0 = (P)p[5], beta = (P)p[6], t0 = (P)p[7];
-      const P time = (P)step;
-      P rx, ry1, ry2, rz, rr1, bzl, omega, ee0;
-      rx = xg + hf - xc; ry1 = yg - yc1; ry2 = yg - yc2; rz = zg - zc;
-      rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
-      bzl = loop_bz(bnorm, rx, ry1, ry2, rz);
-      omega = exp(-rr1 / r0) * beta * cc / r0;
-      ee0 = -omega * bzl * tanh(time / t0);
-      e.x[c] = (S)(rx * ee0);
-      rx = xg - xc; ry1 = yg + hf - yc1; ry2 = yg + hf - yc2; rz = zg - zc;
-      rr1 = sqrt(rx * rx + ry1 * ry1 + rz * rz);
-      bzl = loop_bz(bnorm, rx, ry1, ry2, rz);
-      omega = exp(-rr1 / r0) * beta * cc / r0;
-      ee0 = -omega * bzl * tanh(time / t0);
-      e.y[c] = (S)(ry1 * ee0);
-      rx = xg + hf - xc; ry1 = yg + hf - yc1; ry2 = yg + hf - yc2; rz = zg - zc;
-      b.z[c] = (S)loop_bz(bnorm, rx, ry1, ry2, rz);
+      P ex, ey, bz;
+      loop_driver<P>(p, cc, step_ptr ? *step_ptr : step_val, i, j, k, ex, ey, bz);
+      e.x[c] = (S)ex;
+      e.y[c] = (S)ey;
+      b.z[c] = (S)bz;
     } else {
       const int64_t s = g.at(i, j, 0);
       e.x[c] = e.x[s];
@@ -257,7 +272,7 @@ __global__ void k_bc_loop(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, co
       b.z[c] = b.z[s];
     }
   } else if (mode == 2) {
-    b.z[c] = (S)loop_bz(bnorm, xg + hf - xc, yg + hf - yc1, yg + hf - yc2, zg - zc);
+    b.z[c] = (S)loop_top_bz<P>(p, i, j, k);
     e.x[c] = (S)0;
     e.y[c] = (S)0;
   } else {

@@ -281,15 +281,17 @@ def load(target, p: Problem):
 # --- config: batched parameter scan -------------------------------------------
 def scan(count=48, nmin=64, nmax=192, seed=11, steps=300, cc=0.45, sizes=None):
     """A parameter scan of independent instances with seeded, uneven grid
-    sizes in [nmin, nmax]: a mix of twisted loops (user_loop form, twist
-    angle seeded in [pi/2, 3pi/2]) and vacuum plane waves (user_emwave form,
-    amplitude seeded in [1e-3, 1e-2], seeded in-plane mode)."""
+    sizes in [nmin, nmax]: a mix of driven twisted loops (user_loop field,
+    footpoint twist angle seeded in [pi/2, 3pi/2]) and vacuum plane waves
+    (user_emwave form, amplitude seeded in [1e-3, 1e-2], seeded in-plane
+    mode)."""
     rng = np.random.default_rng(seed)
     out = []
     for q in range(count):
         n = int(sizes[q]) if sizes is not None else int(rng.integers(nmin, nmax + 1))
         if rng.random() < 0.5:
-            p = loop(n, twist_angle=float(rng.uniform(0.5 * math.pi, 1.5 * math.pi)), steps=steps, cc=cc)
+            p = loop(n, twist_angle=float(rng.uniform(0.5 * math.pi, 1.5 * math.pi)), steps=steps, cc=cc,
+                     driver=LOOP_DRIVEN)
         else:
             mode = (int(rng.integers(1, 4)), int(rng.integers(0, 4)))
             p = emwave(n, amp=float(10 ** rng.uniform(-3, -2)), mode=mode, cc=cc, steps=steps)

@@ -21,7 +21,8 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
-LIB_PATH = os.path.join(PKG, "_build", "libtff.so")
+# TFF_LIBRARY selects another build of the library (e.g. a variant under test)
+LIB_PATH = os.environ.get("TFF_LIBRARY") or os.path.join(PKG, "_build", "libtff.so")
 HEADER = os.path.join(ROOT, "include", "tff.h")
 NG = 2
 

@@ -17,7 +17,8 @@ the FP32-bulk path's error against the FP64 path on the same steps.
              300 steps, on the batched path, and the same instances run one
              after another on the single-meshblock path.
 
-Per workload: cell-updates/s of both paths, the energy history's relative
+Per workload: cell-updates/s of each path; for FP32-bulk and for FP32 (the
+reference's own precision), against FP64: the energy history's relative
 error (on E_EM and on E_EM - E_EM(t=0)), the normwise max and L2 relative
 error of every component after the last step, and max |div B|.  One JSON
 object per workload is printed, and appended to --out if given.
@@ -130,22 +131,28 @@ def closed_form_err(p, res, amp, mode, steps):
 
 
 def single_workload(name, p, steps, every, extra=None):
+    """FP64, FP32-bulk and FP32 (the reference's own precision) on the same
+    steps; the two FP32 paths are each compared with FP64."""
     log(f"{name}: {p.n[0]}^3, {steps} steps")
-    r64 = run_single(p, "fp64", steps, every, True)
-    log(f"  fp64 {r64['cups']:.4g} cell-updates/s")
-    r32 = run_single(p, "fp32_bulk", steps, every, True)
-    log(f"  fp32_bulk {r32['cups']:.4g} cell-updates/s")
+    runs = {}
+    for prec in ("fp64", "fp32_bulk", "fp32"):
+        runs[prec] = run_single(p, prec, steps, every, True)
+        log(f"  {prec} {runs[prec]['cups']:.4g} cell-updates/s")
+    r64 = runs["fp64"]
     out = {"workload": name, "n": list(p.n), "steps": steps, "boundary": list(p.boundary), "cc": p.cc,
-           "cells": p.cells, "g": p.g,
-           "fp64": {"cups": r64["cups"], "ms_per_step": r64["ms_per_step"], "tuning": r64["tuning"]},
-           "fp32_bulk": {"cups": r32["cups"], "ms_per_step": r32["ms_per_step"], "tuning": r32["tuning"]},
-           "fp32_bulk_over_fp64": r32["cups"] / r64["cups"]}
-    word = {"fp64": 8, "fp32_bulk": 4}
-    for prec, r in (("fp64", r64), ("fp32_bulk", r32)):
-        out[prec]["roofline_frac_canonical"] = r["cups"] * word[prec] * (48 + 9 * p.g) / 8.0e12
-    out.update(compare(p, r64, r32))
+           "cells": p.cells, "g": p.g}
+    word = {"fp64": 8, "fp32_bulk": 4, "fp32": 4}
+    for prec, r in runs.items():
+        out[prec] = {"cups": r["cups"], "ms_per_step": r["ms_per_step"], "tuning": r["tuning"],
+                     "roofline_frac_canonical": r["cups"] * word[prec] * (48 + 9 * p.g) / 8.0e12}
+    out["fp32_bulk_over_fp64"] = runs["fp32_bulk"]["cups"] / r64["cups"]
+    out["fp32_bulk_vs_fp64"] = compare(p, r64, runs["fp32_bulk"])
+    out["fp32_vs_fp64"] = compare(p, r64, runs["fp32"])
+    h = {prec: (r["hist"][:, 0] + r["hist"][:, 1]).tolist() for prec, r in runs.items()}
+    h["sample_every"] = every
+    out["energy_history"] = h
     if extra:
-        out.update(extra(p, r64, r32))
+        out.update(extra(p, runs))
     return out
 
 
@@ -218,9 +225,9 @@ def main(argv=None) -> int:
             amp, mode = 1e-2, (1, 1)
             p = problems.emwave(64, amp=amp, mode=mode, cc=0.5)
             ns = steps(200)
-            res = single_workload(w, p, ns, 10, lambda p, r64, r32: {
-                "closed_form_err_rel_amp": {"fp64": closed_form_err(p, r64, amp, mode, ns),
-                                            "fp32_bulk": closed_form_err(p, r32, amp, mode, ns)}})
+            res = single_workload(w, p, ns, 10, lambda p, runs: {
+                "closed_form_err_rel_amp": {prec: closed_form_err(p, r, amp, mode, ns)
+                                            for prec, r in runs.items()}})
         elif w == "alfven256":
             res = single_workload(w, problems.alfven(256, seed=1, kmax=8), steps(500), 50)
         elif w == "loop384":
