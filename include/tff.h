@@ -40,9 +40,11 @@ extern "C" {
  * CMakeLists.txt:23,25) */
 #define TFF_FP64 0      /* FP64 storage and arithmetic */
 #define TFF_FP32 1      /* FP32 everywhere, operation order of the reference */
-#define TFF_FP32_BULK 2 /* FP32 storage and stencils; FP64 for |B|^2
-                           normalisation, the E.B projection, the stage
-                           accumulation and all reductions */
+#define TFF_FP32_BULK 2 /* FP32 storage and arithmetic with fused
+                           multiply-adds; the stage update in the form
+                           U^n + c ((X - U^n) + dU), exact in scale; cells
+                           near a magnetic null redo their projection and
+                           currents in FP64; FP64 reductions */
 
 /* boundary flags, src/objects/domain.F90:31-33 and initialize.F90:139-148 */
 #define TFF_OPEN 0

@@ -10,6 +10,7 @@
    wrong index or swapped operand fails it.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -46,6 +47,32 @@ def test_oracle_bitwise_equals_reference_solver(dtype, boundary, n):
         r.clean_epar()
     for f in FIELDS:
         assert np.array_equal(o.field(f), r.get(f), equal_nan=True), f
+
+
+def test_oracle_bitwise_equals_reference_golden_vectors():
+    """The same pin from committed vectors (tests/golden/make_golden.py wrote
+    them with the reference's compiled solver): holds where oracle/_ref is
+    not built."""
+    path = os.path.join(os.path.dirname(__file__), "golden", "ref_field_solver.npz")
+    z = np.load(path)
+    tags = sorted({k.split("/")[0] for k in z.files if not k.startswith("in_")})
+    assert len(tags) == 6
+    for tag in tags:
+        name, bnd = tag.split("_")
+        dtype = np.float64 if name == "f64" else np.float32
+        boundary = tuple(int(c) for c in bnd)
+        o = Oracle((6, 5, 4), boundary=boundary, cc=0.45, absorb_buff=2, dtype=dtype)
+        for f in FIELDS:
+            key = f"in_{bnd}/{f}"
+            if key in z.files:
+                o.field(f)[...] = z[key].astype(dtype)
+        o.init_rk_step()
+        co = rk_coefficients(dtype)
+        for st in range(3):
+            o.rk3_step(*co[3 * st:3 * st + 3])
+            o.clean_epar()
+        for f in FIELDS6:
+            assert np.array_equal(o.field(f), z[f"{tag}/out/{f}"], equal_nan=True), (tag, f)
 
 
 # ---------------------------------------------------------- 2. closed forms

@@ -37,9 +37,14 @@ struct V3 {
 // Precision policies: S = storage, T = stencil arithmetic (differences and
 // interpolation sums), Q = normalisation by |B|^2, E.B projection and
 // limiter, P = stage accumulation, the curl terms and boundary geometry.
-// With null_fp64, a cell whose interpolated |B|^2 is small next to its own
-// |B|^2 (a magnetic null, where the interpolation cancels) redoes the
-// interpolation, normalisation and projection in FP64.
+// With null_fp64 (the FP32-bulk policy), arithmetic contracts into fused
+// multiply-adds, divides through the hardware reciprocal and updates in the
+// scale-exact form of update_cell; a cell whose interpolated |B|^2 is small
+// next to its own |B|^2 (a magnetic null, where the interpolation cancels)
+// redoes the interpolation, projection and currents in FP64.  TFF_BULK_P
+// selects FP64 accumulation for that policy instead (a build option for
+// comparisons; FP32 with the scale-exact update is as accurate and
+// faster).
 struct PrecF64 {
   using S = double;
   using T = double;
