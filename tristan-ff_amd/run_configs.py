@@ -17,8 +17,10 @@ the FP32-bulk path's error against the FP64 path on the same steps.
              300 steps, on the batched path, and the same instances run one
              after another on the single-meshblock path.
 
-Per workload: cell-updates/s of each path; for FP32-bulk and for FP32 (the
-reference's own precision), against FP64: the energy history's relative
+Per workload: cell-updates/s of each path; for FP32-bulk, for FP32 (the
+reference's own precision) and for FP64 started from B perturbed by one
+part in 1e12 (how much the workload itself amplifies roundoff), against
+FP64: the energy history's relative
 error (on E_EM and on E_EM - E_EM(t=0)), the normwise max and L2 relative
 error of every component after the last step, and max |div B|.  One JSON
 object per workload is printed, and appended to --out if given.
@@ -26,6 +28,7 @@ object per workload is printed, and appended to --out if given.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import math
 import os
@@ -138,6 +141,12 @@ def single_workload(name, p, steps, every, extra=None):
     for prec in ("fp64", "fp32_bulk", "fp32"):
         runs[prec] = run_single(p, prec, steps, every, True)
         log(f"  {prec} {runs[prec]['cups']:.4g} cell-updates/s")
+    # FP64 again from B perturbed by one part in 1e12: how far roundoff-size
+    # differences grow on this workload by themselves
+    pp = dataclasses.replace(p, fields=dict(p.fields))
+    pp.fields["bz"] = p.fields["bz"] * (1.0 + 1e-12)
+    rpert = run_single(pp, "fp64", steps, every, True)
+    log(f"  fp64 with B perturbed by 1e-12: done")
     r64 = runs["fp64"]
     out = {"workload": name, "n": list(p.n), "steps": steps, "boundary": list(p.boundary), "cc": p.cc,
            "cells": p.cells, "g": p.g}
@@ -148,7 +157,9 @@ def single_workload(name, p, steps, every, extra=None):
     out["fp32_bulk_over_fp64"] = runs["fp32_bulk"]["cups"] / r64["cups"]
     out["fp32_bulk_vs_fp64"] = compare(p, r64, runs["fp32_bulk"])
     out["fp32_vs_fp64"] = compare(p, r64, runs["fp32"])
+    out["fp64_perturbed_1e-12_vs_fp64"] = compare(p, r64, rpert)
     h = {prec: (r["hist"][:, 0] + r["hist"][:, 1]).tolist() for prec, r in runs.items()}
+    h["fp64_perturbed_1e-12"] = (rpert["hist"][:, 0] + rpert["hist"][:, 1]).tolist()
     h["sample_every"] = every
     out["energy_history"] = h
     if extra:
