@@ -1085,6 +1085,10 @@ int tff_stage_kernel_time(tff_handle* h, double* total_ms, long* launches) {
 // Batched path: independent meshblocks advanced together, one persistent
 // queue-driven launch of the fused stage kernel per RK stage.
 
+// block size of the batched ghost passes (their per-instance lists are
+// padded to it)
+constexpr int BATCH_TB = 256;
+
 struct tff_batch {
   int precision = TFF_FP32_BULK;
   int device = 0;
@@ -1203,15 +1207,18 @@ int batch_build_t(tff_batch* b) {
     a.nblocks = a.nbx * a.nby * ((h->g.nz + a.kchunk - 1) / a.kchunk);
     tile_base[q] = tt;
     tt += a.nblocks;
+    // per-instance lists padded to whole blocks of the ghost passes, so
+    // that a block never spans two instances
+    auto pad = [](int64_t n) { return (n + BATCH_TB - 1) / BATCH_TB * BATCH_TB; };
     if (!wrap) {
       shell_base.push_back(ss);
       shell_inst.push_back(q);
-      ss += shell_count(h);
+      ss += pad(shell_count(h));
     }
     if (h->bc_kind == TFF_BC_LOOP && h->bc_host[8] == 2.0) {
       floor_base.push_back(ff);
       floor_inst.push_back(q);
-      ff += (int64_t)h->g.nx * h->g.ny;
+      ff += pad((int64_t)h->g.nx * h->g.ny);
     }
     for (int w = 0; w < 4; ++w) {
       BatchStage<S, P>& x = st[(size_t)w * n + q];
@@ -1326,7 +1333,7 @@ int batch_ghosts(tff_batch* b, int which) {
   if (!b->nghost) return TFF_OK;
   const int n = (int)b->inst.size();
   const BatchStage<S, P>* st = (const BatchStage<S, P>*)b->d_stage + (size_t)which * n;
-  const int tb = 256;
+  const int tb = BATCH_TB;
   if (b->nfloor)
     hipLaunchKernelGGL(k_batch_floor<PR>, dim3((unsigned)((b->total_floor + tb - 1) / tb)), dim3(tb), 0, b->stream,
                        st, (const BatchInfo*)b->d_info, b->d_floor_base, b->d_floor_inst, b->nfloor,

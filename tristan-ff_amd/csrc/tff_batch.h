@@ -77,14 +77,15 @@ __global__ void k_batch_ghost_build(const BatchStage<typename PR::S, typename PR
   using P = typename PR::P;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int q = find_item(shell_base, nghost, t);
+  // each instance's list starts on a block boundary: one lookup per block
+  const int q = find_item(shell_base, nghost, (int64_t)blockIdx.x * blockDim.x);
   const int inst = shell_inst[q];
   const BatchStage<S, P>& b = st[inst];
   const BatchInfo& in = info[inst];
   const Geom& g = b.a.g;
   const int64_t cs = g.cs;
   int i, j, k;
-  shell_cell(g, t - shell_base[q], i, j, k);
+  if (!shell_cell(g, t - shell_base[q], i, j, k)) return;
   const int64_t d = g.at(i, j, k);
   S* oe = b.a.oe;
   S* ob = b.a.ob;
@@ -158,12 +159,13 @@ __global__ void k_batch_floor(const BatchStage<typename PR::S, typename PR::P>* 
   using P = typename PR::P;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int q = find_item(floor_base, nfloor, t);
+  const int q = find_item(floor_base, nfloor, (int64_t)blockIdx.x * blockDim.x);
   const int inst = floor_inst[q];
   const BatchStage<S, P>& b = st[inst];
   const Geom& g = b.a.g;
-  const int64_t r = t - floor_base[q];
-  const int i = (int)(r % g.nx), j = (int)(r / g.nx);
+  const uint32_t r = (uint32_t)(t - floor_base[q]);
+  if (r >= (uint32_t)g.nx * (uint32_t)g.ny) return;
+  const int i = (int)(r % (uint32_t)g.nx), j = (int)(r / (uint32_t)g.nx);
   P ex, ey, bz;
   loop_driver<P>(info[inst].bcp, b.a.cc, *step_ptr, i, j, 0, ex, ey, bz);
   const int64_t c = g.at(i, j, 0);
@@ -183,14 +185,14 @@ __global__ void k_batch_ghost_clean(const BatchStage<typename PR::S, typename PR
   using P = typename PR::P;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int q = find_item(shell_base, nghost, t);
+  const int q = find_item(shell_base, nghost, (int64_t)blockIdx.x * blockDim.x);
   const int inst = shell_inst[q];
   const BatchStage<S, P>& b = st[inst];
   const BatchInfo& in = info[inst];
   const Geom& g = b.a.g;
   const int64_t cs = g.cs;
   int i, j, k;
-  shell_cell(g, t - shell_base[q], i, j, k);
+  if (!shell_cell(g, t - shell_base[q], i, j, k)) return;
   const int64_t d = g.at(i, j, k);
   const S* ye = b.a.oe;
   const S* yb = b.a.ob;

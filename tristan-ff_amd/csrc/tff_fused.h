@@ -11,18 +11,18 @@
 // along z over `kchunk` planes.  Planes live in LDS ring buffers:
 //   raw E, B     with a 2-cell halo in x and y (the clean of the 1-cell halo
 //                needs its own neighbours),
-//   cleaned E    with a 1-cell halo (the stencil of rk3Push),
-//   rho          on the (TX+1) x (TY+1) nodes the currents need.
+//   cleaned E    with a 1-cell halo (the stencil of rk3Push).
 // Each ring keeps the planes its readers still need while the next raw
 // plane is stored: ex, ey raw and cleaned 2, ez raw and cleaned 3, bz 3,
-// bx and by 4 (the curl of plane k reads plane k-1), rho 2.
+// bx and by 4 (the curl of plane k reads plane k-1).
 //
-// Per iteration (output plane k): store raw plane k+2 (loaded into
-// registers during the previous iteration) and issue the loads of plane k+3
-// and of U^n at plane k, so that their latency overlaps this iteration's
-// work; clean plane k+1 on the halo, rho on plane k+1, then rk3Push /
-// rk3Update on plane k.  The per-cell arithmetic is tff_cell.h's, so
-// results equal the unfused kernels bit for bit.
+// Per iteration (output plane k), three barriers: store raw plane k+2
+// (loaded into registers during the previous iteration) and issue the loads
+// of plane k+3 and of U^n at plane k, so that their latency overlaps this
+// iteration's work | clean plane k+1 on the tile and its halo | rk3Push /
+// rk3Update on plane k, with rho evaluated at the four nodes each cell's
+// currents read | ring slots free.  The per-cell arithmetic is
+// tff_cell.h's, so results equal the unfused kernels bit for bit.
 //
 // Addressing: every thread precomputes the in-plane (x-y) element offsets of
 // the cells it loads or writes once per tile; per plane only a uniform base
@@ -42,12 +42,11 @@ struct Tile {
   static constexpr int TX = TX_, TY = TY_;
   static constexpr int W2 = TX + 4, H2 = TY + 4, N2 = W2 * H2;
   static constexpr int W1 = TX + 2, H1 = TY + 2, N1 = W1 * H1;
-  static constexpr int WN = TX + 1, HN = TY + 1, NN = WN * HN;
   static constexpr int NT = TX * TY;
   static constexpr int L2PT = (N2 + NT - 1) / NT;
   template <class S, class T>
   static constexpr size_t lds_bytes() {
-    return sizeof(S) * (size_t)(18 * N2 + 7 * N1) + sizeof(T) * (size_t)(2 * NN);
+    return sizeof(S) * (size_t)(18 * N2 + 7 * N1);
   }
 };
 
@@ -109,6 +108,7 @@ __device__ __forceinline__ int s2(int p) { return p & 1; }
 __device__ __forceinline__ int s3(int p) { return (p + 6) % 3; }
 __device__ __forceinline__ int s4(int p) { return p & 3; }
 
+
 // One tile (tile_id in x-fastest, then y, then z-chunk order) of a stage.
 template <class PR, class TL>
 __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typename PR::P>& A, int tile_id,
@@ -118,7 +118,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
   using Q = typename PR::Q;
   using P = typename PR::P;
   constexpr int TX = TL::TX, TY = TL::TY, W2 = TL::W2, N2 = TL::N2, W1 = TL::W1, N1 = TL::N1;
-  constexpr int WN = TL::WN, NN = TL::NN, NT = TL::NT, L2PT = TL::L2PT;
+  constexpr int NT = TL::NT, L2PT = TL::L2PT;
   S* rex = (S*)smem_raw;         // [2][N2]
   S* rey = rex + 2 * N2;         // [2][N2]
   S* rez = rey + 2 * N2;         // [3][N2]
@@ -128,7 +128,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
   S* cex = rbz + 3 * N2;         // [2][N1]
   S* cey = cex + 2 * N1;         // [2][N1]
   S* cez = cey + 2 * N1;         // [3][N1]
-  T* rho = (T*)(cez + 3 * N1);   // [2][NN]
 
   const Geom& g = A.g;
   const int64_t cs = g.cs;
@@ -279,21 +278,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
     }
   };
 
-  // computeRho of plane q on the nodes of the tile and its +x, +y edge
-  auto rho_plane = [&](int q) {
-    for (int c = tid; c < NN; c += NT) {
-      const int nx_ = c % WN, ny_ = c / WN;
-      const int lc = (ny_ + 1) * W1 + (nx_ + 1);
-      const auto lex = lds_cell<S, W1>(cex, N1, q, lc, s2), ley = lds_cell<S, W1>(cey, N1, q, lc, s2);
-      const auto lez = lds_cell<S, W1>(cez, N1, q, lc, s3);
-      auto EX = [&](int a, int b, int d) { return (T)lex(a, b, d); };
-      auto EY = [&](int a, int b, int d) { return (T)ley(a, b, d); };
-      auto EZ = [&](int a, int b, int d) { return (T)lez(a, b, d); };
-      rho[s2(q) * NN + c] = rho_node<T>(EX, EY, EZ, 0, 0, 0);
-    }
-  };
-
-  // prologue: planes k0-2 .. k0+1 raw, k0-1 and k0 cleaned, rho at k0
+  // prologue: planes k0-2 .. k0+1 raw, k0-1 and k0 cleaned
   load_plane(k0 - 2);
   store_plane(k0 - 2);
   load_plane(k0 - 1);
@@ -308,7 +293,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
   __syncthreads();
   clean_plane(k0);
   __syncthreads();
-  rho_plane(k0);
   cbb = nbb;
   cbref = nbref;
   if (k0 < k1) load_plane(k0 + 2);
@@ -318,7 +302,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
   const bool active = gi < g.nx && gj < g.ny;
   const int lc1 = (ty + 1) * W1 + (tx + 1);
   const int lc2 = (ty + 2) * W2 + (tx + 2);
-  const int lcn = ty * WN + tx;
   // x-y byte offset of the thread's own cell (clamped for inactive threads)
   const uint32_t oxy =
       (uint32_t)((g.org + min(gi, g.nx - 1) + (int64_t)min(gj, g.ny - 1) * g.px) * (int64_t)sizeof(S));
@@ -356,22 +339,21 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
     __syncthreads();
     clean_plane(k + 1);
     __syncthreads();
-    rho_plane(k + 1);
-    __syncthreads();
     if (active) {
       if (!EARLY) load_un();
       const auto lex = lds_cell<S, W1>(cex, N1, k, lc1, s2), ley = lds_cell<S, W1>(cey, N1, k, lc1, s2);
       const auto lez = lds_cell<S, W1>(cez, N1, k, lc1, s3);
       const auto lbx = lds_cell<S, W2>(rbx, N2, k, lc2, s4), lby = lds_cell<S, W2>(rby, N2, k, lc2, s4);
       const auto lbz = lds_cell<S, W2>(rbz, N2, k, lc2, s3);
-      const auto lrho = lds_cell<T, WN>(rho, NN, k, lcn, s2);
       auto EX = [&](int a, int b, int d) { return (T)lex(a, b, d); };
       auto EY = [&](int a, int b, int d) { return (T)ley(a, b, d); };
       auto EZ = [&](int a, int b, int d) { return (T)lez(a, b, d); };
       auto BXF = [&](int a, int b, int d) { return (T)lbx(a, b, d); };
       auto BYF = [&](int a, int b, int d) { return (T)lby(a, b, d); };
       auto BZF = [&](int a, int b, int d) { return (T)lbz(a, b, d); };
-      auto RHO = [&](int a, int b, int d) { return lrho(a, b, d); };
+      // computeRho at the four nodes the currents read, from the cleaned
+      // rings (cheaper than a separate rho pass and its barrier)
+      auto RHO = [&](int a, int b, int d) { return rho_node<T>(EX, EY, EZ, a, b, d); };
       P d[6];
       if constexpr (REUSE)
         rhs_cell_bb<PR>(EX, EY, EZ, BXF, BYF, BZF, RHO, cbb, cbref, A.cc, tt[0], tt[1], tt[2], d);

@@ -115,11 +115,15 @@ __global__ void __launch_bounds__(BX* BY)
 // ---------------------------------------------------------------------------
 // Ghost shell enumeration: z-slabs (all i, j), then y-slabs (interior k, all
 // i), then x-slabs (interior j, k).  Every ghost cell appears exactly once.
-__device__ __forceinline__ bool shell_cell(const Geom& g, int64_t t, int& i, int& j, int& k) {
-  const int sx = g.nx + 2 * NG, sy = g.ny + 2 * NG;
-  const int64_t nzc = (int64_t)2 * NG * sx * sy;
-  const int64_t nyc = (int64_t)2 * NG * sx * g.nz;
-  const int64_t nxc = (int64_t)2 * NG * g.ny * g.nz;
+// A shell has far fewer than 2^32 cells, so the index arithmetic is 32-bit
+// (64-bit division is a long software sequence on the GPU).
+__device__ __forceinline__ bool shell_cell(const Geom& g, int64_t t64, int& i, int& j, int& k) {
+  const uint32_t sx = (uint32_t)(g.nx + 2 * NG), sy = (uint32_t)(g.ny + 2 * NG);
+  const uint32_t nz = (uint32_t)g.nz, ny = (uint32_t)g.ny;
+  const uint32_t nzc = 2u * NG * sx * sy;
+  const uint32_t nyc = 2u * NG * sx * nz;
+  const uint32_t nxc = 2u * NG * ny * nz;
+  uint32_t t = (uint32_t)t64;
   if (t < nzc) {
     i = (int)(t % sx) - NG;
     t /= sx;
@@ -132,17 +136,17 @@ __device__ __forceinline__ bool shell_cell(const Geom& g, int64_t t, int& i, int
   if (t < nyc) {
     i = (int)(t % sx) - NG;
     t /= sx;
-    k = (int)(t % g.nz);
-    const int jj = (int)(t / g.nz);
+    k = (int)(t % nz);
+    const int jj = (int)(t / nz);
     j = jj < NG ? jj - NG : g.ny + jj - NG;
     return true;
   }
   t -= nyc;
   if (t < nxc) {
-    const int ii = (int)(t % (2 * NG));
-    t /= (2 * NG);
-    j = (int)(t % g.ny);
-    k = (int)(t / g.ny);
+    const int ii = (int)(t % (2u * NG));
+    t /= (2u * NG);
+    j = (int)(t % ny);
+    k = (int)(t / ny);
     i = ii < NG ? ii - NG : g.nx + ii - NG;
     return true;
   }
@@ -251,9 +255,10 @@ __global__ void k_bc_loop(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, co
   const int sx = g.nx + 2 * NG, sy = g.ny + 2 * NG;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)(2 * NG + 1) * sx * sy) return;
-  const int i = (int)(t % sx) - NG;
-  const int j = (int)((t / sx) % sy) - NG;
-  const int kk = (int)(t / ((int64_t)sx * sy));
+  const uint32_t u = (uint32_t)t;  // < 5 planes of cells: 32-bit arithmetic
+  const int i = (int)(u % (uint32_t)sx) - NG;
+  const int j = (int)((u / (uint32_t)sx) % (uint32_t)sy) - NG;
+  const int kk = (int)(u / ((uint32_t)sx * (uint32_t)sy));
   const int k = kk < NG ? kk - NG : (kk == NG ? 0 : g.nz + kk - NG - 1);
   const int mode = (int)p[8];
   if (k == 0 && mode != 2) return;
@@ -294,9 +299,10 @@ __global__ void k_bc_uniform(Geom g, V3<typename PR::S> e, V3<typename PR::S> b,
   const int sx = g.nx + 2 * NG, sy = g.ny + 2 * NG;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)(2 * NG + 1) * sx * sy) return;
-  const int i = (int)(t % sx) - NG;
-  const int j = (int)((t / sx) % sy) - NG;
-  const int kk = (int)(t / ((int64_t)sx * sy));
+  const uint32_t u = (uint32_t)t;  // < 5 planes of cells: 32-bit arithmetic
+  const int i = (int)(u % (uint32_t)sx) - NG;
+  const int j = (int)((u / (uint32_t)sx) % (uint32_t)sy) - NG;
+  const int kk = (int)(u / ((uint32_t)sx * (uint32_t)sy));
   const int k = kk <= NG ? kk - NG : g.nz + kk - NG - 1;
   const int64_t c = g.at(i, j, k);
   const int step = step_ptr ? *step_ptr : step_val;
@@ -368,9 +374,10 @@ __global__ void k_bc_psr_eval(Geom g, V3<const typename PR::S> e, V3<const typen
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nbox = (int64_t)n.x * n.y * n.z;
   if (t >= nbox) return;
-  const int i = lo.x + (int)(t % n.x);
-  const int j = lo.y + (int)((t / n.x) % n.y);
-  const int k = lo.z + (int)(t / ((int64_t)n.x * n.y));
+  const uint32_t u = (uint32_t)t;  // the bounding box of r < 2R: 32-bit arithmetic
+  const int i = lo.x + (int)(u % (uint32_t)n.x);
+  const int j = lo.y + (int)((u / (uint32_t)n.x) % (uint32_t)n.y);
+  const int k = lo.z + (int)(u / ((uint32_t)n.x * (uint32_t)n.y));
   const P R = (P)p[0], om = (P)p[2], xc = (P)p[3], yc = (P)p[4], zc = (P)p[5];
   const P ssq = (R * (P)2) * (R * (P)2);
   const P fi = (P)i, fj = (P)j, fk = (P)k;
@@ -485,9 +492,10 @@ __global__ void k_bc_psr_commit(Geom g, V3<typename PR::S> e, V3<typename PR::S>
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nbox = (int64_t)n.x * n.y * n.z;
   if (t >= nbox) return;
-  const int i = lo.x + (int)(t % n.x);
-  const int j = lo.y + (int)((t / n.x) % n.y);
-  const int k = lo.z + (int)(t / ((int64_t)n.x * n.y));
+  const uint32_t u = (uint32_t)t;  // the bounding box of r < 2R: 32-bit arithmetic
+  const int i = lo.x + (int)(u % (uint32_t)n.x);
+  const int j = lo.y + (int)((u / (uint32_t)n.x) % (uint32_t)n.y);
+  const int k = lo.z + (int)(u / ((uint32_t)n.x * (uint32_t)n.y));
   const P R = (P)p[0], xc = (P)p[3], yc = (P)p[4], zc = (P)p[5];
   const P ssq = (R * (P)2) * (R * (P)2);
   const P fi = (P)i, fj = (P)j, fk = (P)k;
