@@ -21,25 +21,78 @@ struct BBar {
   Q2 x[3], y[3], z[3], m[3];
 };
 
+// The B interpolants at E point P alone (the expressions bbar uses).
+template <int P, class T2, class BXa, class BYa, class BZa>
+__device__ __forceinline__ void bbar_point(const BXa& BXF, const BYa& BYF, const BZa& BZF, T2& bx, T2& by,
+                                           T2& bz) {
+  const T2 h = (T2)0.5, e = (T2)0.125;
+  if constexpr (P == 0) {
+    bx = e * ((T2)BXF(0, 0, 0) + (T2)BXF(0, -1, 0) + (T2)BXF(1, -1, 0) + (T2)BXF(1, 0, 0) +
+              (T2)BXF(0, 0, -1) + (T2)BXF(0, -1, -1) + (T2)BXF(1, -1, -1) + (T2)BXF(1, 0, -1));
+    by = h * ((T2)BYF(0, 0, 0) + (T2)BYF(0, 0, -1));
+    bz = h * ((T2)BZF(0, 0, 0) + (T2)BZF(0, -1, 0));
+  } else if constexpr (P == 1) {
+    bx = h * ((T2)BXF(0, 0, 0) + (T2)BXF(0, 0, -1));
+    by = e * ((T2)BYF(0, 0, 0) + (T2)BYF(-1, 0, 0) + (T2)BYF(-1, 1, 0) + (T2)BYF(0, 1, 0) +
+              (T2)BYF(0, 0, -1) + (T2)BYF(-1, 0, -1) + (T2)BYF(-1, 1, -1) + (T2)BYF(0, 1, -1));
+    bz = h * ((T2)BZF(0, 0, 0) + (T2)BZF(-1, 0, 0));
+  } else {
+    bx = h * ((T2)BXF(0, 0, 0) + (T2)BXF(0, -1, 0));
+    by = h * ((T2)BYF(0, 0, 0) + (T2)BYF(-1, 0, 0));
+    bz = e * ((T2)BZF(0, 0, 0) + (T2)BZF(-1, 0, 0) + (T2)BZF(-1, -1, 0) + (T2)BZF(0, -1, 0) +
+              (T2)BZF(0, 0, 1) + (T2)BZF(-1, 0, 1) + (T2)BZF(-1, -1, 1) + (T2)BZF(0, -1, 1));
+  }
+}
+
 template <class T2, class Q2, bool FAST, class BXa, class BYa, class BZa>
 __device__ __forceinline__ BBar<Q2> bbar(const BXa& BXF, const BYa& BYF, const BZa& BZF) {
-  const T2 h = (T2)0.5, e = (T2)0.125;
   BBar<Q2> r;
-  r.x[0] = (T2)(e * ((T2)BXF(0, 0, 0) + (T2)BXF(0, -1, 0) + (T2)BXF(1, -1, 0) + (T2)BXF(1, 0, 0) +
-                     (T2)BXF(0, 0, -1) + (T2)BXF(0, -1, -1) + (T2)BXF(1, -1, -1) + (T2)BXF(1, 0, -1)));
-  r.y[0] = (T2)(h * ((T2)BYF(0, 0, 0) + (T2)BYF(0, 0, -1)));
-  r.z[0] = (T2)(h * ((T2)BZF(0, 0, 0) + (T2)BZF(0, -1, 0)));
-  r.x[1] = (T2)(h * ((T2)BXF(0, 0, 0) + (T2)BXF(0, 0, -1)));
-  r.y[1] = (T2)(e * ((T2)BYF(0, 0, 0) + (T2)BYF(-1, 0, 0) + (T2)BYF(-1, 1, 0) + (T2)BYF(0, 1, 0) +
-                     (T2)BYF(0, 0, -1) + (T2)BYF(-1, 0, -1) + (T2)BYF(-1, 1, -1) + (T2)BYF(0, 1, -1)));
-  r.z[1] = (T2)(h * ((T2)BZF(0, 0, 0) + (T2)BZF(-1, 0, 0)));
-  r.x[2] = (T2)(h * ((T2)BXF(0, 0, 0) + (T2)BXF(0, -1, 0)));
-  r.y[2] = (T2)(h * ((T2)BYF(0, 0, 0) + (T2)BYF(-1, 0, 0)));
-  r.z[2] = (T2)(e * ((T2)BZF(0, 0, 0) + (T2)BZF(-1, 0, 0) + (T2)BZF(-1, -1, 0) + (T2)BZF(0, -1, 0) +
-                     (T2)BZF(0, 0, 1) + (T2)BZF(-1, 0, 1) + (T2)BZF(-1, -1, 1) + (T2)BZF(0, -1, 1)));
+  T2 x, y, z;
+  bbar_point<0, T2>(BXF, BYF, BZF, x, y, z);
+  r.x[0] = x; r.y[0] = y; r.z[0] = z;
+  bbar_point<1, T2>(BXF, BYF, BZF, x, y, z);
+  r.x[1] = x; r.y[1] = y; r.z[1] = z;
+  bbar_point<2, T2>(BXF, BYF, BZF, x, y, z);
+  r.x[2] = x; r.y[2] = y; r.z[2] = z;
 #pragma unroll
   for (int p = 0; p < 3; ++p) r.m[p] = sq3<FAST>(r.x[p], r.y[p], r.z[p]);
   return r;
+}
+
+// E interpolated at E point P: its own component as is, the other two as
+// 4-point averages (field_solver.F90:93-95, :116-118, :139-141; cleanEpar
+// the same, :288-328).
+template <int P, class T2, class AX, class AY, class AZ>
+__device__ __forceinline__ void e_point(const AX& EX, const AY& EY, const AZ& EZ, T2& ex, T2& ey, T2& ez) {
+  const T2 q = (T2)0.25;
+  if constexpr (P == 0) {
+    ex = (T2)EX(0, 0, 0);
+    ey = q * ((T2)EY(0, 0, 0) + (T2)EY(1, 0, 0) + (T2)EY(0, -1, 0) + (T2)EY(1, -1, 0));
+    ez = q * ((T2)EZ(0, 0, 0) + (T2)EZ(1, 0, 0) + (T2)EZ(0, 0, -1) + (T2)EZ(1, 0, -1));
+  } else if constexpr (P == 1) {
+    ex = q * ((T2)EX(0, 0, 0) + (T2)EX(-1, 0, 0) + (T2)EX(0, 1, 0) + (T2)EX(-1, 1, 0));
+    ey = (T2)EY(0, 0, 0);
+    ez = q * ((T2)EZ(0, 0, 0) + (T2)EZ(0, 1, 0) + (T2)EZ(0, 0, -1) + (T2)EZ(0, 1, -1));
+  } else {
+    ex = q * ((T2)EX(0, 0, 0) + (T2)EX(-1, 0, 0) + (T2)EX(0, 0, 1) + (T2)EX(-1, 0, 1));
+    ey = q * ((T2)EY(0, 0, 0) + (T2)EY(0, -1, 0) + (T2)EY(0, 0, 1) + (T2)EY(0, -1, 1));
+    ez = (T2)EZ(0, 0, 0);
+  }
+}
+
+// cleanEpar of E point P in FP64 from the raw fields (the near-null redo,
+// one point at a time to keep its register footprint small)
+template <int P, class AX, class AY, class AZ, class BXa, class BYa, class BZa>
+__device__ __forceinline__ double clean_point_fp64(const AX& EX, const AY& EY, const AZ& EZ, const BXa& BXF,
+                                                   const BYa& BYF, const BZa& BZF, int guard) {
+  double ex, ey, ez, bx, by, bz;
+  e_point<P, double>(EX, EY, EZ, ex, ey, ez);
+  bbar_point<P, double>(BXF, BYF, BZF, bx, by, bz);
+  const double bmag = sq3<false>(bx, by, bz);
+  const double epar = ex * bx + ey * by + ez * bz;
+  const double e0 = P == 0 ? ex : (P == 1 ? ey : ez);
+  const double b0 = P == 0 ? bx : (P == 1 ? by : bz);
+  return (guard && bmag == 0.0) ? e0 : e0 - epar * b0 / bmag;
 }
 
 // cleanEpar at one cell (field_solver.F90:288-328) with sums in T2 and the
@@ -51,12 +104,10 @@ __device__ __forceinline__ BBar<Q2> bbar(const BXa& BXF, const BYa& BYF, const B
 template <class T2, class Q2, bool FAST, class AX, class AY, class AZ>
 __device__ __forceinline__ void clean_impl(const AX& EX, const AY& EY, const AZ& EZ, const BBar<Q2>& bb,
                                            int guard, Q2 bref, Q2& ox, Q2& oy, Q2& oz, bool& near) {
-  const T2 q = (T2)0.25;
   near = false;
   {
-    T2 intex = (T2)EX(0, 0, 0);
-    T2 intey = q * ((T2)EY(0, 0, 0) + (T2)EY(1, 0, 0) + (T2)EY(0, -1, 0) + (T2)EY(1, -1, 0));
-    T2 intez = q * ((T2)EZ(0, 0, 0) + (T2)EZ(1, 0, 0) + (T2)EZ(0, 0, -1) + (T2)EZ(1, 0, -1));
+    T2 intex, intey, intez;
+    e_point<0, T2>(EX, EY, EZ, intex, intey, intez);
     const Q2 bx_ = bb.x[0], by_ = bb.y[0], bz_ = bb.z[0], bmag = bb.m[0];
     Q2 epar = FAST ? mad_<FAST>((Q2)intez, bz_, mad_<FAST>((Q2)intey, by_, (Q2)intex * bx_))
                    : ((Q2)intex * bx_ + (Q2)intey * by_ + (Q2)intez * bz_);
@@ -65,9 +116,8 @@ __device__ __forceinline__ void clean_impl(const AX& EX, const AY& EY, const AZ&
     ox = (guard && bmag == (Q2)0) ? e0 : (FAST ? e0 - div_<FAST>(epar * bx_, bmag) : e0 - epar * bx_ / bmag);
   }
   {
-    T2 intex = q * ((T2)EX(0, 0, 0) + (T2)EX(-1, 0, 0) + (T2)EX(0, 1, 0) + (T2)EX(-1, 1, 0));
-    T2 intey = (T2)EY(0, 0, 0);
-    T2 intez = q * ((T2)EZ(0, 0, 0) + (T2)EZ(0, 1, 0) + (T2)EZ(0, 0, -1) + (T2)EZ(0, 1, -1));
+    T2 intex, intey, intez;
+    e_point<1, T2>(EX, EY, EZ, intex, intey, intez);
     const Q2 bx_ = bb.x[1], by_ = bb.y[1], bz_ = bb.z[1], bmag = bb.m[1];
     Q2 epar = FAST ? mad_<FAST>((Q2)intez, bz_, mad_<FAST>((Q2)intey, by_, (Q2)intex * bx_))
                    : ((Q2)intex * bx_ + (Q2)intey * by_ + (Q2)intez * bz_);
@@ -76,9 +126,8 @@ __device__ __forceinline__ void clean_impl(const AX& EX, const AY& EY, const AZ&
     oy = (guard && bmag == (Q2)0) ? e0 : (FAST ? e0 - div_<FAST>(epar * by_, bmag) : e0 - epar * by_ / bmag);
   }
   {
-    T2 intex = q * ((T2)EX(0, 0, 0) + (T2)EX(-1, 0, 0) + (T2)EX(0, 0, 1) + (T2)EX(-1, 0, 1));
-    T2 intey = q * ((T2)EY(0, 0, 0) + (T2)EY(0, -1, 0) + (T2)EY(0, 0, 1) + (T2)EY(0, -1, 1));
-    T2 intez = (T2)EZ(0, 0, 0);
+    T2 intex, intey, intez;
+    e_point<2, T2>(EX, EY, EZ, intex, intey, intez);
     const Q2 bx_ = bb.x[2], by_ = bb.y[2], bz_ = bb.z[2], bmag = bb.m[2];
     Q2 epar = FAST ? mad_<FAST>((Q2)intez, bz_, mad_<FAST>((Q2)intey, by_, (Q2)intex * bx_))
                    : ((Q2)intex * bx_ + (Q2)intey * by_ + (Q2)intez * bz_);
@@ -109,13 +158,9 @@ __device__ __forceinline__ void clean_cell_bb(const AX& EX, const AY& EY, const 
   bool near;
   clean_impl<T, Q, PR::null_fp64>(EX, EY, EZ, bb, guard, bref, qx, qy, qz, near);
   if (PR::null_fp64 && near) {
-    double dx, dy, dz;
-    bool nn;
-    const BBar<double> b64 = bbar<double, double, false>(BXF, BYF, BZF);
-    clean_impl<double, double, false>(EX, EY, EZ, b64, guard, 0.0, dx, dy, dz, nn);
-    ox = (P)dx;
-    oy = (P)dy;
-    oz = (P)dz;
+    ox = (P)clean_point_fp64<0>(EX, EY, EZ, BXF, BYF, BZF, guard);
+    oy = (P)clean_point_fp64<1>(EX, EY, EZ, BXF, BYF, BZF, guard);
+    oz = (P)clean_point_fp64<2>(EX, EY, EZ, BXF, BYF, BZF, guard);
     return;
   }
   ox = (P)qx;
@@ -149,32 +194,48 @@ template <class T2, class Q2, bool FAST, class AX, class AY, class AZ>
 __device__ __forceinline__ void currents_impl(const AX& EX, const AY& EY, const AZ& EZ, const BBar<Q2>& bb,
                                               Q2 cc, T2 r000, T2 r100, T2 r010, T2 r001, Q2 bref, Q2* j,
                                               bool& near) {
-  const T2 h = (T2)0.5, q = (T2)0.25;
+  const T2 h = (T2)0.5;
   near = false;
   {  // X current (:90-111)
     T2 intrho = h * (r100 + r000);
-    T2 intex = (T2)EX(0, 0, 0);
-    T2 intey = q * ((T2)EY(0, 0, 0) + (T2)EY(1, 0, 0) + (T2)EY(0, -1, 0) + (T2)EY(1, -1, 0));
-    T2 intez = q * ((T2)EZ(0, 0, 0) + (T2)EZ(1, 0, 0) + (T2)EZ(0, 0, -1) + (T2)EZ(1, 0, -1));
+    T2 intex, intey, intez;
+    e_point<0, T2>(EX, EY, EZ, intex, intey, intez);
     j[0] = drift_current<Q2, FAST>(cc, (Q2)intrho, (Q2)intex, (Q2)intey, (Q2)intez, bb.x[0], bb.y[0],
                                    bb.z[0], bb.m[0], 0, bref, near);
   }
   {  // Y current (:113-134)
     T2 intrho = h * (r010 + r000);
-    T2 intex = q * ((T2)EX(0, 0, 0) + (T2)EX(-1, 0, 0) + (T2)EX(0, 1, 0) + (T2)EX(-1, 1, 0));
-    T2 intey = (T2)EY(0, 0, 0);
-    T2 intez = q * ((T2)EZ(0, 0, 0) + (T2)EZ(0, 1, 0) + (T2)EZ(0, 0, -1) + (T2)EZ(0, 1, -1));
+    T2 intex, intey, intez;
+    e_point<1, T2>(EX, EY, EZ, intex, intey, intez);
     j[1] = drift_current<Q2, FAST>(cc, (Q2)intrho, (Q2)intey, (Q2)intex, (Q2)intez, bb.y[1], bb.x[1],
                                    bb.z[1], bb.m[1], 1, bref, near);
   }
   {  // Z current (:136-157)
     T2 intrho = h * (r000 + r001);
-    T2 intex = q * ((T2)EX(0, 0, 0) + (T2)EX(-1, 0, 0) + (T2)EX(0, 0, 1) + (T2)EX(-1, 0, 1));
-    T2 intey = q * ((T2)EY(0, 0, 0) + (T2)EY(0, -1, 0) + (T2)EY(0, 0, 1) + (T2)EY(0, -1, 1));
-    T2 intez = (T2)EZ(0, 0, 0);
+    T2 intex, intey, intez;
+    e_point<2, T2>(EX, EY, EZ, intex, intey, intez);
     j[2] = drift_current<Q2, FAST>(cc, (Q2)intrho, (Q2)intez, (Q2)intex, (Q2)intey, bb.z[2], bb.x[2],
                                    bb.y[2], bb.m[2], 2, bref, near);
   }
+}
+
+// Drift current at E point P in FP64 from the raw fields (the near-null
+// redo of currents_impl, one component at a time)
+template <int P, class AX, class AY, class AZ, class BXa, class BYa, class BZa>
+__device__ __forceinline__ double current_point_fp64(const AX& EX, const AY& EY, const AZ& EZ, const BXa& BXF,
+                                                     const BYa& BYF, const BZa& BZF, double cc,
+                                                     double intrho) {
+  double ex, ey, ez, bx, by, bz;
+  e_point<P, double>(EX, EY, EZ, ex, ey, ez);
+  bbar_point<P, double>(BXF, BYF, BZF, bx, by, bz);
+  const double bmag = sq3<false>(bx, by, bz);
+  bool nn = false;
+  if constexpr (P == 0)
+    return drift_current<double, false>(cc, intrho, ex, ey, ez, bx, by, bz, bmag, 0, 0.0, nn);
+  else if constexpr (P == 1)
+    return drift_current<double, false>(cc, intrho, ey, ex, ez, by, bx, bz, bmag, 1, 0.0, nn);
+  else
+    return drift_current<double, false>(cc, intrho, ez, ex, ey, bz, bx, by, bmag, 2, 0.0, nn);
 }
 
 // rk3Push at one cell (field_solver.F90:81-162) given the B interpolants
@@ -190,12 +251,8 @@ __device__ __forceinline__ void rhs_cell_bb(const AX& EX, const AY& EY, const AZ
   using T = typename PR::T;
   using Q = typename PR::Q;
   using P = typename PR::P;
-  P dbx = cc * (P)((EY(0, 0, 1) - EY(0, 0, 0)) - (EZ(0, 1, 0) - EZ(0, 0, 0)));
-  P dby = cc * (P)((EZ(1, 0, 0) - EZ(0, 0, 0)) - (EX(0, 0, 1) - EX(0, 0, 0)));
-  P dbz = cc * (P)((EX(0, 1, 0) - EX(0, 0, 0)) - (EY(1, 0, 0) - EY(0, 0, 0)));
-  P dex = -cc * (P)(((BYF(0, 0, 0) - BYF(0, 0, -1)) - (BZF(0, 0, 0) - BZF(0, -1, 0))) - tx);
-  P dey = -cc * (P)(((BZF(0, 0, 0) - BZF(-1, 0, 0)) - (BXF(0, 0, 0) - BXF(0, 0, -1))) - ty);
-  P dez = -cc * (P)(((BXF(0, 0, 0) - BXF(0, -1, 0)) - (BYF(0, 0, 0) - BYF(-1, 0, 0))) - tz);
+  // the currents first: their FP64 redo then runs with the fewest other
+  // values live
   const T r000 = RHO(0, 0, 0);
   const T r100 = RHO(1, 0, 0);
   const T r010 = RHO(0, 1, 0);
@@ -203,20 +260,25 @@ __device__ __forceinline__ void rhs_cell_bb(const AX& EX, const AY& EY, const AZ
   Q j[3];
   bool near;
   currents_impl<T, Q, PR::null_fp64>(EX, EY, EZ, bb, (Q)cc, r000, r100, r010, r001, bref, j, near);
+  P jx, jy, jz;
   if (PR::null_fp64 && near) {
-    double jd[3];
-    bool nn;
-    const BBar<double> b64 = bbar<double, double, false>(BXF, BYF, BZF);
-    currents_impl<double, double, false>(EX, EY, EZ, b64, (double)cc, (double)r000, (double)r100,
-                                         (double)r010, (double)r001, 0.0, jd, nn);
-    dex = dex - (P)jd[0];
-    dey = dey - (P)jd[1];
-    dez = dez - (P)jd[2];
+    // FP64 redo, one current at a time (small register footprint)
+    const double c = (double)cc, h = 0.5;
+    const double q000 = (double)r000;
+    jx = (P)current_point_fp64<0>(EX, EY, EZ, BXF, BYF, BZF, c, h * ((double)r100 + q000));
+    jy = (P)current_point_fp64<1>(EX, EY, EZ, BXF, BYF, BZF, c, h * ((double)r010 + q000));
+    jz = (P)current_point_fp64<2>(EX, EY, EZ, BXF, BYF, BZF, c, h * (q000 + (double)r001));
   } else {
-    dex = dex - (P)j[0];
-    dey = dey - (P)j[1];
-    dez = dez - (P)j[2];
+    jx = (P)j[0];
+    jy = (P)j[1];
+    jz = (P)j[2];
   }
+  const P dbx = cc * (P)((EY(0, 0, 1) - EY(0, 0, 0)) - (EZ(0, 1, 0) - EZ(0, 0, 0)));
+  const P dby = cc * (P)((EZ(1, 0, 0) - EZ(0, 0, 0)) - (EX(0, 0, 1) - EX(0, 0, 0)));
+  const P dbz = cc * (P)((EX(0, 1, 0) - EX(0, 0, 0)) - (EY(1, 0, 0) - EY(0, 0, 0)));
+  const P dex = -cc * (P)(((BYF(0, 0, 0) - BYF(0, 0, -1)) - (BZF(0, 0, 0) - BZF(0, -1, 0))) - tx) - jx;
+  const P dey = -cc * (P)(((BZF(0, 0, 0) - BZF(-1, 0, 0)) - (BXF(0, 0, 0) - BXF(0, 0, -1))) - ty) - jy;
+  const P dez = -cc * (P)(((BXF(0, 0, 0) - BXF(0, -1, 0)) - (BYF(0, 0, 0) - BYF(-1, 0, 0))) - tz) - jz;
   d[0] = dex; d[1] = dey; d[2] = dez;
   d[3] = dbx; d[4] = dby; d[5] = dbz;
 }
