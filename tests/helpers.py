@@ -41,3 +41,32 @@ def field_errors(get, ref):
         den = max(np.max(np.abs(np.asarray(ref(v + c), np.float64))) for c in "xyz")
         out[v.upper()] = num / den if den > 0 else num
     return out
+
+
+def near_null_cells(f):
+    """Interior cells whose B interpolant at one of the three E points has
+    |B|^2 below 1e-4 times the cell's own |B|^2 (or below 1e-30): the cells
+    at which the FP32-bulk kernels redo the clean and the currents in FP64.
+    `f` maps bx, by, bz to arrays with ghosts, indexed [k][j][i]."""
+    bx, by, bz = (np.asarray(interior(f[c]), np.float64) for c in ("bx", "by", "bz"))
+
+    def at(a, di=0, dj=0, dk=0):  # a at (i + di, j + dj, k + dk), periodic
+        return np.roll(a, (-dk, -dj, -di), axis=(0, 1, 2))
+
+    bref = bx * bx + by * by + bz * bz
+    # Ex point: bx over (i, i+1) x (j-1, j) x (k-1, k), by over k-1, k; bz over j-1, j
+    x0 = 0.125 * sum(at(bx, di, dj, dk) for di in (0, 1) for dj in (-1, 0) for dk in (-1, 0))
+    y0 = 0.5 * (at(by) + at(by, 0, 0, -1))
+    z0 = 0.5 * (at(bz) + at(bz, 0, -1))
+    # Ey point
+    x1 = 0.5 * (at(bx) + at(bx, 0, 0, -1))
+    y1 = 0.125 * sum(at(by, di, dj, dk) for di in (-1, 0) for dj in (0, 1) for dk in (-1, 0))
+    z1 = 0.5 * (at(bz) + at(bz, -1))
+    # Ez point
+    x2 = 0.5 * (at(bx) + at(bx, 0, -1))
+    y2 = 0.5 * (at(by) + at(by, -1))
+    z2 = 0.125 * sum(at(bz, di, dj, dk) for di in (-1, 0) for dj in (-1, 0) for dk in (0, 1))
+    near = np.zeros(bx.shape, bool)
+    for m in (x0 * x0 + y0 * y0 + z0 * z0, x1 * x1 + y1 * y1 + z1 * z1, x2 * x2 + y2 * y2 + z2 * z2):
+        near |= (m < 1e-4 * bref) | (m < 1e-30)
+    return near

@@ -52,6 +52,26 @@ def test_fused_equals_unfused_bitwise(name, precision):
     print("tuning", b.get_tuning())
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_bulk"])
+@pytest.mark.parametrize("name", ["loop_driven", "psr", "uniform"])
+def test_multi_step_graphs_keep_the_step(name, precision):
+    """tff_run replays graphs of several steps whose hooks read the step
+    from a device counter plus their offset in the graph: 19 steps (an eager
+    step, two 8-step graphs and one-step graphs) of step-dependent boundary
+    drivers equal the per-procedure path, step by step."""
+    p = CASES[name]()
+    print(f"case {name} {precision} n={p.n}", flush=True)
+    a = _mb(p, precision, fused=False)
+    b = _mb(p, precision, fused=True)
+    for s in range(19):
+        a.step(s)
+    b.run(0, 19)
+    dt = np.float64 if precision == "fp64" else np.float32
+    for f in FIELDS6:
+        x, y = a.download(f, dt), b.download(f, dt)
+        assert np.array_equal(x, y, equal_nan=True), (f, float(np.nanmax(np.abs(x - y))))
+
+
 @pytest.mark.parametrize("name", ["alfven", "loop", "psr"])
 def test_fused_fp64_matches_oracle(name):
     p = CASES[name]()

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -54,6 +55,10 @@ struct tff_handle {
   bool t0_dirty = true;
   int has_t0 = 0;
   int* d_flag = nullptr;
+  // FP32-bulk fused path: one word per stage-kernel tile marking the planes
+  // it left to k_stage_fix (room for fix_cap tiles)
+  unsigned long long* d_fixmask = nullptr;
+  int64_t fix_cap = 0;
 
   int bc_kind = TFF_BC_NONE;
   double bc_host[16] = {0};
@@ -83,7 +88,8 @@ struct tff_handle {
   int kchunk = 0;
   int max_lds = 65536;
 
-  std::map<std::pair<void*, void*>, hipGraphExec_t> graphs;  // keyed by the state/clean E buffers
+  // keyed by the state and clean E buffers and the number of steps captured
+  std::map<std::tuple<void*, void*, int>, hipGraphExec_t> graphs;
   std::vector<void*> allocs;
 };
 
@@ -323,6 +329,9 @@ using TileB = Tile<32, 8>;
 using TileC = Tile<64, 4>;
 using TileD = Tile<64, 8>;
 constexpr int NTILES = 4;
+// steps per captured graph in tff_run (fewer graph launches and step-counter
+// updates; a remainder replays the one-step graph)
+constexpr int RUN_GRAPH_STEPS = 8;
 constexpr int TILE_TX[NTILES] = {32, 32, 64, 64};
 constexpr int TILE_TY[NTILES] = {4, 8, 4, 8};
 
@@ -350,6 +359,16 @@ int64_t shell_count(const tff_handle* h) {
   return (int64_t)2 * NG * sx * sy + (int64_t)2 * NG * sx * h->g.nz + (int64_t)2 * NG * h->g.ny * h->g.nz;
 }
 
+// the tile grid of a TL launch: x fastest, then y, then z-chunks
+template <class TL, class SA>
+SA with_tile_grid(const tff_handle* h, const SA& a) {
+  SA b = a;
+  b.nbx = (h->g.nx + TL::TX - 1) / TL::TX;
+  b.nby = (h->g.ny + TL::TY - 1) / TL::TY;
+  b.nblocks = b.nbx * b.nby * ((h->g.nz + a.kchunk - 1) / a.kchunk);
+  return b;
+}
+
 template <class PR, class TL>
 int launch_stage_t(tff_handle* h, const StageArgs<typename PR::S, typename PR::P>& a) {
   using S = typename PR::S;
@@ -360,13 +379,29 @@ int launch_stage_t(tff_handle* h, const StageArgs<typename PR::S, typename PR::P
     hipFuncSetAttribute((const void*)k_stage<PR, TL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr = true;
   }
-  StageArgs<typename PR::S, typename PR::P> b = a;
-  b.nbx = (h->g.nx + TL::TX - 1) / TL::TX;
-  b.nby = (h->g.ny + TL::TY - 1) / TL::TY;
-  b.nblocks = b.nbx * b.nby * ((h->g.nz + a.kchunk - 1) / a.kchunk);
+  const auto b = with_tile_grid<TL>(h, a);
+  if (PR::null_fp64 && (!b.fixmask || b.nblocks > h->fix_cap))
+    return fail(h, TFF_ESTATE, "near-null plane masks missing or too small");
   const unsigned grid = (unsigned)((b.nblocks + 7) / 8 * 8);
   hipLaunchKernelGGL((k_stage<PR, TL>), dim3(grid), dim3(TL::NT), lds, h->stream, b);
   return last_launch(h, "k_stage");
+}
+
+// k_stage_fix after a k_stage<PR, TL> launch: one block per CU scans the
+// tiles' plane masks (a few words each) and recomputes what they mark.
+constexpr unsigned FIX_GRID = 256;
+template <class PR, class TL>
+int launch_fix_t(tff_handle* h, const StageArgs<typename PR::S, typename PR::P>& a) {
+  using S = typename PR::S;
+  using T = typename PR::T;
+  const size_t lds = TL::template lds_bytes<S, T>();
+  static bool attr = false;
+  if (!attr) {
+    hipFuncSetAttribute((const void*)k_stage_fix<PR, TL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_stage_fix<PR, TL>), dim3(FIX_GRID), dim3(TL::NT), lds, h->stream, with_tile_grid<TL>(h, a));
+  return last_launch(h, "k_stage_fix");
 }
 
 template <class PR>
@@ -391,7 +426,15 @@ int launch_stage(tff_handle* h, const StageArgs<typename PR::S, typename PR::P>&
   }
   if (rc) return rc;
   if (h->timing) HIPCHK(h, hipEventRecord(e1, h->stream));
-  return TFF_OK;
+  if constexpr (PR::null_fp64) {
+    switch (tile) {
+      case 0: rc = launch_fix_t<PR, TileA>(h, a); break;
+      case 1: rc = launch_fix_t<PR, TileB>(h, a); break;
+      case 2: rc = launch_fix_t<PR, TileC>(h, a); break;
+      default: rc = launch_fix_t<PR, TileD>(h, a); break;
+    }
+  }
+  return rc;
 }
 
 template <class PR>
@@ -414,6 +457,7 @@ StageArgs<typename PR::S, typename PR::P> stage_common(tff_handle* h) {
   a.guard = h->cfg.absorb_fixed;
   a.wrap = wrap_mode(h) ? 1 : 0;
   a.kchunk = h->kchunk;
+  a.fixmask = h->d_fixmask;
   return a;
 }
 
@@ -590,9 +634,14 @@ int autotune_t(tff_handle* h) {
   return TFF_OK;
 }
 
+// One timestep, or with use_graph up to `count` steps from one captured
+// graph; *done gets the number of steps enqueued.  Under a graph the hooks
+// read the step from d_step plus their step's offset in the graph, and one
+// k_bump_step at the end advances d_step by the steps captured.
 template <class PR>
 struct DoStep {
-  static int run(tff_handle* h, int step, int use_graph) {
+  static int run(tff_handle* h, int step, int use_graph, int count, int* done) {
+    *done = 1;
     int rc = ensure_t0<PR>(h);
     if (rc) return rc;
     if (h->fused) {
@@ -601,7 +650,7 @@ struct DoStep {
       // steady state has fixed buffer roles and is replayed from one graph
       if (!use_graph || h->timing || h->state_clean) {
         if ((rc = enqueue_fused_step<PR>(h, nullptr, step))) return rc;
-        if (use_graph) hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(64), 0, h->stream, h->d_step);
+        if (use_graph) hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(64), 0, h->stream, h->d_step, 1);
         return last_launch(h, "k_bump_step");
       }
     } else {
@@ -610,17 +659,19 @@ struct DoStep {
         rc = enqueue_step<PR>(h, nullptr, step);
         if (rc) return rc;
         std::swap(h->es, h->ce);
-        if (use_graph) hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(64), 0, h->stream, h->d_step);
+        if (use_graph) hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(64), 0, h->stream, h->d_step, 1);
         return last_launch(h, "k_bump_step");
       }
+      count = 1;  // the buffer roles alternate from step to step
     }
-    auto key = std::make_pair(h->es.p[0], h->ce.p[0]);
+    auto key = std::make_tuple(h->es.p[0], h->ce.p[0], count);
     auto it = h->graphs.find(key);
     if (it == h->graphs.end()) {
       hipGraph_t graph;
       HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      rc = h->fused ? enqueue_fused_step<PR>(h, h->d_step, 0) : enqueue_step<PR>(h, h->d_step, 0);
-      if (rc == TFF_OK) hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(64), 0, h->stream, h->d_step);
+      for (int s = 0; s < count && rc == TFF_OK; ++s)
+        rc = h->fused ? enqueue_fused_step<PR>(h, h->d_step, s) : enqueue_step<PR>(h, h->d_step, s);
+      if (rc == TFF_OK) hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(64), 0, h->stream, h->d_step, count);
       hipError_t e = hipStreamEndCapture(h->stream, &graph);
       if (rc) return rc;
       if (e != hipSuccess) return fail(h, TFF_EHIP, "hipStreamEndCapture failed");
@@ -632,6 +683,7 @@ struct DoStep {
     }
     HIPCHK(h, hipGraphLaunch(it->second, h->stream));
     if (!h->fused) std::swap(h->es, h->ce);
+    *done = count;
     return TFF_OK;
   }
 };
@@ -772,6 +824,11 @@ int tff_create(const tff_config* cfg, tff_handle** out) {
               hipMalloc(&h->d_out, 4 * sizeof(double)) != hipSuccess))
     rc = TFF_ENOMEM;
   if (!rc && hipMemsetAsync(h->d_bc, 0, 16 * sizeof(double), h->stream) != hipSuccess) rc = TFF_EHIP;
+  if (!rc && cfg->precision == TFF_FP32_BULK) {
+    // the most tiles any tile shape and z-chunk can make (32 x 4, one plane)
+    h->fix_cap = (int64_t)((cfg->nx + 31) / 32) * ((cfg->ny + 3) / 4) * cfg->nz;
+    if (hipMalloc(&h->d_fixmask, (size_t)h->fix_cap * sizeof(unsigned long long)) != hipSuccess) rc = TFF_ENOMEM;
+  }
   if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = TFF_EHIP;
   for (int s = 0; !rc && s < 16; ++s)
     if (hipEventCreate(&h->ev[s]) != hipSuccess) rc = TFF_EHIP;
@@ -791,6 +848,7 @@ void tff_destroy(tff_handle* h) {
   if (h->psr_stage) hipFree(h->psr_stage);
   hipFree(h->d_step);
   hipFree(h->d_flag);
+  if (h->d_fixmask) hipFree(h->d_fixmask);
   hipFree(h->d_bc);
   hipFree(h->d_part);
   hipFree(h->d_out);
@@ -968,7 +1026,8 @@ int tff_user_bc(tff_handle* h, int step) {
 
 int tff_step(tff_handle* h, int timestep) {
   if (!h) return TFF_EINVAL;
-  return dispatch<DoStep>(h, timestep, 0);
+  int done = 0;
+  return dispatch<DoStep>(h, timestep, 0, 1, &done);
 }
 
 int tff_run(tff_handle* h, int first_step, int nsteps) {
@@ -976,9 +1035,12 @@ int tff_run(tff_handle* h, int first_step, int nsteps) {
   if (nsteps == 0) return TFF_OK;
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->d_step, first_step);
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  for (int n = 0; n < nsteps; ++n) {
-    int rc = dispatch<DoStep>(h, first_step + n, 1);
+  for (int n = 0; n < nsteps;) {
+    int done = 0;
+    const int count = nsteps - n >= RUN_GRAPH_STEPS ? RUN_GRAPH_STEPS : 1;
+    int rc = dispatch<DoStep>(h, first_step + n, 1, count, &done);
     if (rc) return rc;
+    n += done;
   }
   return TFF_OK;
 }
@@ -1032,7 +1094,8 @@ int tff_set_path(tff_handle* h, int fused) {
 
 int tff_set_tuning(tff_handle* h, int tile, int kchunk) {
   if (!h) return TFF_EINVAL;
-  if (tile < -1 || tile >= NTILES || (tile >= 0 && kchunk < 1)) return fail(h, TFF_EINVAL, "bad tuning");
+  if (tile < -1 || tile >= NTILES || (tile >= 0 && (kchunk < 1 || kchunk > 64)))
+    return fail(h, TFF_EINVAL, "bad tuning (tile index out of range, or z-chunk not in 1..64)");
   if (tile >= 0) {
     size_t lds = 0;
     dispatch<TileLds>(h, tile, &lds);
@@ -1359,7 +1422,7 @@ int batch_enqueue_step(tff_batch* b) {
     if ((rc = batch_launch_stage<PR>(b, w))) return rc;
     if ((rc = batch_ghosts<PR>(b, w))) return rc;
   }
-  hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(64), 0, b->stream, b->d_step);
+  hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(64), 0, b->stream, b->d_step, 1);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return bfail(b, TFF_EHIP, hipGetErrorString(e));
   return TFF_OK;

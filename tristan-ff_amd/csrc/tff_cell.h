@@ -80,6 +80,17 @@ __device__ __forceinline__ void e_point(const AX& EX, const AY& EY, const AZ& EZ
   }
 }
 
+// 1/x in FP64 from the hardware estimate and two Newton steps (relative
+// error ~1e-16; the near-null redo needs FP64 range and precision, not the
+// correctly rounded quotient, and the full division sequence would set the
+// register footprint of the whole FP32-bulk kernel)
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  return r;
+}
+
 // cleanEpar of E point P in FP64 from the raw fields (the near-null redo,
 // one point at a time to keep its register footprint small)
 template <int P, class AX, class AY, class AZ, class BXa, class BYa, class BZa>
@@ -92,7 +103,8 @@ __device__ __forceinline__ double clean_point_fp64(const AX& EX, const AY& EY, c
   const double epar = ex * bx + ey * by + ez * bz;
   const double e0 = P == 0 ? ex : (P == 1 ? ey : ez);
   const double b0 = P == 0 ? bx : (P == 1 ? by : bz);
-  return (guard && bmag == 0.0) ? e0 : e0 - epar * b0 / bmag;
+  if (guard && bmag == 0.0) return e0;
+  return e0 - epar * b0 * rcp_nr(bmag);
 }
 
 // cleanEpar at one cell (field_solver.F90:288-328) with sums in T2 and the
@@ -144,9 +156,21 @@ __device__ __forceinline__ Q2 cell_b2(const BXa& BXF, const BYa& BYF, const BZa&
   return a * a + b * b + c * c;
 }
 
+// Whether the FP64 redo of clean_cell_bb or rhs_cell_bb can apply at a cell:
+// the union of their two tests over the three E points.
+template <class Q>
+__device__ __forceinline__ bool near_null_cell(const BBar<Q>& bb, Q bref) {
+  bool r = false;
+#pragma unroll
+  for (int p = 0; p < 3; ++p) r |= bb.m[p] < (Q)NEAR_NULL * bref || bb.m[p] < (Q)1e-30f;
+  return r;
+}
+
 // cleanEpar at one cell given its B interpolants `bb` and raw |B|^2 `bref`
 // (0 unless PR::null_fp64); a near-null point redoes everything in FP64.
-template <class PR, class AX, class AY, class AZ, class BXa, class BYa, class BZa>
+// REDO = false leaves out the redo (a caller that recomputes near-null cells
+// elsewhere; near_null_cell names them).
+template <class PR, bool REDO = PR::null_fp64, class AX, class AY, class AZ, class BXa, class BYa, class BZa>
 __device__ __forceinline__ void clean_cell_bb(const AX& EX, const AY& EY, const AZ& EZ, const BXa& BXF,
                                               const BYa& BYF, const BZa& BZF, const BBar<typename PR::Q>& bb,
                                               typename PR::Q bref, int guard, typename PR::P& ox,
@@ -157,7 +181,7 @@ __device__ __forceinline__ void clean_cell_bb(const AX& EX, const AY& EY, const 
   Q qx, qy, qz;
   bool near;
   clean_impl<T, Q, PR::null_fp64>(EX, EY, EZ, bb, guard, bref, qx, qy, qz, near);
-  if (PR::null_fp64 && near) {
+  if (REDO && near) {
     ox = (P)clean_point_fp64<0>(EX, EY, EZ, BXF, BYF, BZF, guard);
     oy = (P)clean_point_fp64<1>(EX, EY, EZ, BXF, BYF, BZF, guard);
     oz = (P)clean_point_fp64<2>(EX, EY, EZ, BXF, BYF, BZF, guard);
@@ -228,21 +252,32 @@ __device__ __forceinline__ double current_point_fp64(const AX& EX, const AY& EY,
   double ex, ey, ez, bx, by, bz;
   e_point<P, double>(EX, EY, EZ, ex, ey, ez);
   bbar_point<P, double>(BXF, BYF, BZF, bx, by, bz);
+  // the limiter and the drift current of drift_current, with the components
+  // named by axis
   const double bmag = sq3<false>(bx, by, bz);
-  bool nn = false;
+  const double emag = sq3<false>(ex, ey, ez);
+  if (emag > bmag) {
+    const double corr = sqrt(bmag / emag);  // rare branch: IEEE
+    if (P != 0) ex *= corr;
+    if (P != 1) ey *= corr;
+    if (P != 2) ez *= corr;
+  }
+  double cross;
   if constexpr (P == 0)
-    return drift_current<double, false>(cc, intrho, ex, ey, ez, bx, by, bz, bmag, 0, 0.0, nn);
+    cross = ey * bz - by * ez;
   else if constexpr (P == 1)
-    return drift_current<double, false>(cc, intrho, ey, ex, ez, by, bx, bz, bmag, 1, 0.0, nn);
+    cross = ez * bx - ex * bz;
   else
-    return drift_current<double, false>(cc, intrho, ez, ex, ey, bz, bx, by, bmag, 2, 0.0, nn);
+    cross = ex * by - bx * ey;
+  return cc * (intrho * cross) * rcp_nr(bmag + 1e-7);  // TINY, src/defs.F90:4
 }
 
 // rk3Push at one cell (field_solver.F90:81-162) given the B interpolants
 // `bb` and raw |B|^2 `bref` of the cell: d[0..2] = dE, d[3..5] = dB.
 // RHO(a, b, d) gives the charge density at node (i+a, j+b, k+d); t0 is the
 // precomputed B0 curl bracket (zero for a uniform background).
-template <class PR, class AX, class AY, class AZ, class BXa, class BYa, class BZa, class RF>
+template <class PR, bool REDO = PR::null_fp64, class AX, class AY, class AZ, class BXa, class BYa, class BZa,
+          class RF>
 __device__ __forceinline__ void rhs_cell_bb(const AX& EX, const AY& EY, const AZ& EZ, const BXa& BXF,
                                             const BYa& BYF, const BZa& BZF, const RF& RHO,
                                             const BBar<typename PR::Q>& bb, typename PR::Q bref,
@@ -261,7 +296,7 @@ __device__ __forceinline__ void rhs_cell_bb(const AX& EX, const AY& EY, const AZ
   bool near;
   currents_impl<T, Q, PR::null_fp64>(EX, EY, EZ, bb, (Q)cc, r000, r100, r010, r001, bref, j, near);
   P jx, jy, jz;
-  if (PR::null_fp64 && near) {
+  if (REDO && near) {
     // FP64 redo, one current at a time (small register footprint)
     const double c = (double)cc, h = 0.5;
     const double q000 = (double)r000;

@@ -46,7 +46,7 @@ struct Tile {
   static constexpr int L2PT = (N2 + NT - 1) / NT;
   template <class S, class T>
   static constexpr size_t lds_bytes() {
-    return sizeof(S) * (size_t)(18 * N2 + 7 * N1);
+    return sizeof(S) * (size_t)(18 * N2 + 7 * N1) + 4 * sizeof(unsigned);  // rings + near-null flags
   }
 };
 
@@ -69,6 +69,9 @@ struct StageArgs {
   Absorb<P> ab;
   int has_t0, stage1, clean_in, guard, wrap, kchunk;
   int nbx, nby, nblocks;  // tile grid and its size
+  // near-null deferral (FP32-bulk): per tile, bit k - k0 marks output
+  // plane k as left to k_stage_fix (every tile writes its word)
+  unsigned long long* fixmask;
 };
 
 __device__ __forceinline__ int wrap_index(int v, int n) {
@@ -109,10 +112,19 @@ __device__ __forceinline__ int s3(int p) { return (p + 6) % 3; }
 __device__ __forceinline__ int s4(int p) { return p & 3; }
 
 
-// One tile (tile_id in x-fastest, then y, then z-chunk order) of a stage.
-template <class PR, class TL>
-__device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typename PR::P>& A, int tile_id,
-                                           unsigned char* smem_raw) {
+// Planes k0..k1-1 of the TX x TY column at (i0, j0) of a stage.
+//
+// FIX (FP32-bulk only) moves the near-null FP64 redo out of this kernel:
+// the clean marks, per plane, whether any cell whose B interpolants it
+// evaluates can take the redo (near_null_cell), and an output plane that
+// reads such a plane (k-1, k or k+1) is skipped and marked in the tile's
+// word of A.fixmask for k_stage_fix, which recomputes it with the redo.
+// Skipped cells are not written, so the in-place stage 3 leaves their U^n
+// for the fix-up to read.  Without the redo code the kernel fits four waves
+// per SIMD.
+template <class PR, class TL, bool FIX>
+__device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typename PR::P>& A, int i0, int j0,
+                                          int k0, int k1, int tile_id, unsigned char* smem_raw) {
   using S = typename PR::S;
   using T = typename PR::T;
   using Q = typename PR::Q;
@@ -128,16 +140,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
   S* cex = rbz + 3 * N2;         // [2][N1]
   S* cey = cex + 2 * N1;         // [2][N1]
   S* cez = cey + 2 * N1;         // [3][N1]
+  unsigned* nflag = (unsigned*)(cez + 3 * N1);  // [4]: plane q in slot q & 3
 
   const Geom& g = A.g;
   const int64_t cs = g.cs;
   const int tid = threadIdx.x;
-  const int bxi = tile_id % A.nbx;
-  const int byi = (tile_id / A.nbx) % A.nby;
-  const int bzi = tile_id / (A.nbx * A.nby);
-  const int i0 = bxi * TX, j0 = byi * TY;
-  const int k0 = bzi * A.kchunk;
-  const int k1 = min(k0 + A.kchunk, g.nz);
+  if (FIX && tid < 4) nflag[tid] = 0u;
 
   // x-y byte offsets (from the plane origin) of this thread's halo cells
   // of the raw planes: wrapped on periodic boxes, clamped to the stored
@@ -230,6 +238,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
     if (project || (REUSE && own)) {
       bb = bbar<T, Q, PR::null_fp64>(BXF, BYF, BZF);
       if (PR::null_fp64) bref = cell_b2<Q>(BXF, BYF, BZF);
+      if (FIX && near_null_cell<Q>(bb, bref)) nflag[(unsigned)q & 3u] = 1u;
     }
     if (!A.clean_in) {
       ox = lex(0, 0, 0);
@@ -242,7 +251,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
       oz = A.ge[a + 2 * cs];
     } else {
       P px, py, pz;
-      clean_cell_bb<PR>(EX, EY, EZ, BXF, BYF, BZF, bb, bref, A.guard, px, py, pz);
+      clean_cell_bb<PR, PR::null_fp64 && !FIX>(EX, EY, EZ, BXF, BYF, BZF, bb, bref, A.guard, px, py, pz);
       ox = (S)px;
       oy = (S)py;
       oz = (S)pz;
@@ -306,9 +315,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
   const uint32_t oxy =
       (uint32_t)((g.org + min(gi, g.nx - 1) + (int64_t)min(gj, g.ny - 1) * g.px) * (int64_t)sizeof(S));
 
+  unsigned long long skipped = 0ull;  // FIX: bit k - k0 for a skipped plane k
   for (int k = k0; k < k1; ++k) {
     // raw plane k+2 arrived during the previous iteration
     store_plane(k + 2);
+    // the flag slot of plane k+2 (last read for plane k-2's output)
+    if (FIX && tid == 0) nflag[(unsigned)(k + 2) & 3u] = 0u;
     const int64_t pk = (int64_t)k * g.plane;
     // U^n and the B0 curl of plane k, then raw plane k+3, in flight while
     // this iteration cleans and computes (FP64 loads U^n where it is used:
@@ -339,7 +351,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
     __syncthreads();
     clean_plane(k + 1);
     __syncthreads();
-    if (active) {
+    bool skip = false;
+    if constexpr (FIX) {
+      const unsigned f = nflag[(unsigned)(k - 1) & 3u] | nflag[(unsigned)k & 3u] | nflag[(unsigned)(k + 1) & 3u];
+      skip = __builtin_amdgcn_readfirstlane(f) != 0u;
+      if (skip) skipped |= 1ull << (k - k0);
+    }
+    if (active && !skip) {
       if (!EARLY) load_un();
       const auto lex = lds_cell<S, W1>(cex, N1, k, lc1, s2), ley = lds_cell<S, W1>(cey, N1, k, lc1, s2);
       const auto lez = lds_cell<S, W1>(cez, N1, k, lc1, s3);
@@ -356,7 +374,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
       auto RHO = [&](int a, int b, int d) { return rho_node<T>(EX, EY, EZ, a, b, d); };
       P d[6];
       if constexpr (REUSE)
-        rhs_cell_bb<PR>(EX, EY, EZ, BXF, BYF, BZF, RHO, cbb, cbref, A.cc, tt[0], tt[1], tt[2], d);
+        rhs_cell_bb<PR, PR::null_fp64 && !FIX>(EX, EY, EZ, BXF, BYF, BZF, RHO, cbb, cbref, A.cc, tt[0], tt[1],
+                                               tt[2], d);
       else
         rhs_cell<PR>(EX, EY, EZ, BXF, BYF, BZF, RHO, A.cc, tt[0], tt[1], tt[2], d);
       P v[6] = {(P)lex(0, 0, 0), (P)ley(0, 0, 0), (P)lez(0, 0, 0),
@@ -387,10 +406,22 @@ __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typen
     cbref = nbref;
     __syncthreads();  // plane k's rings are free for plane k+3 / k+2
   }
+  if (FIX && tid == 0) A.fixmask[tile_id] = skipped;
+}
+
+// One tile (tile_id in x-fastest, then y, then z-chunk order) of a stage.
+template <class PR, class TL, bool FIX = false>
+__device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typename PR::P>& A, int tile_id,
+                                           unsigned char* smem_raw) {
+  const int bxi = tile_id % A.nbx;
+  const int byi = (tile_id / A.nbx) % A.nby;
+  const int bzi = tile_id / (A.nbx * A.nby);
+  const int k0 = bzi * A.kchunk;
+  stage_run<PR, TL, FIX>(A, bxi * TL::TX, byi * TL::TY, k0, min(k0 + A.kchunk, A.g.nz), tile_id, smem_raw);
 }
 
 template <class PR, class TL>
-__global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(sizeof(typename PR::S) == 4 ? 3 : 2)))
+__global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(sizeof(typename PR::S) == 4 ? 4 : 2)))
     k_stage(StageArgs<typename PR::S, typename PR::P> A) {
   extern __shared__ unsigned char smem_raw[];
   // Workgroups are dealt to the 8 XCDs round-robin; give each XCD one
@@ -399,7 +430,31 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(siz
   const int per = (A.nblocks + 7) / 8;
   const int tile_id = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
   if (tile_id >= A.nblocks) return;
-  stage_tile<PR, TL>(A, tile_id, smem_raw);
+  stage_tile<PR, TL, PR::null_fp64>(A, tile_id, smem_raw);
+}
+
+// The planes k_stage skipped (FIX), with the near-null FP64 redo: the same
+// per-cell code as the stage kernel of the unfused path, so the results are
+// those of a kernel that redoes inline.  Block b takes tiles b, b + gridDim.x,
+// ... and recomputes each run of marked planes of each.
+template <class PR, class TL>
+__global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(2)))
+    k_stage_fix(StageArgs<typename PR::S, typename PR::P> A) {
+  extern __shared__ unsigned char smem_raw[];
+  for (int t = (int)blockIdx.x; t < A.nblocks; t += (int)gridDim.x) {
+    unsigned long long m = A.fixmask[t];
+    if (m == 0ull) continue;
+    const int i0 = (t % A.nbx) * TL::TX;
+    const int j0 = ((t / A.nbx) % A.nby) * TL::TY;
+    const int k0 = (t / (A.nbx * A.nby)) * A.kchunk;
+    while (m != 0ull) {
+      const int a = __builtin_ctzll(m);                 // first marked plane
+      const unsigned long long rest = ~(m >> a);
+      const int len = rest == 0ull ? 64 - a : __builtin_ctzll(rest);  // marked planes from a on
+      stage_run<PR, TL, false>(A, i0, j0, k0 + a, k0 + a + len, t, smem_raw);
+      m = (a + len >= 64) ? 0ull : (m >> (a + len)) << (a + len);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

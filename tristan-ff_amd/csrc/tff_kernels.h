@@ -266,7 +266,7 @@ __global__ void k_bc_loop(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, co
   if (k <= 0) {
     if (mode != 0) {
       P ex, ey, bz;
-      loop_driver<P>(p, cc, step_ptr ? *step_ptr : step_val, i, j, k, ex, ey, bz);
+      loop_driver<P>(p, cc, step_ptr ? *step_ptr + step_val : step_val, i, j, k, ex, ey, bz);
       e.x[c] = (S)ex;
       e.y[c] = (S)ey;
       b.z[c] = (S)bz;
@@ -305,7 +305,7 @@ __global__ void k_bc_uniform(Geom g, V3<typename PR::S> e, V3<typename PR::S> b,
   const int kk = (int)(u / ((uint32_t)sx * (uint32_t)sy));
   const int k = kk <= NG ? kk - NG : g.nz + kk - NG - 1;
   const int64_t c = g.at(i, j, k);
-  const int step = step_ptr ? *step_ptr : step_val;
+  const int step = step_ptr ? *step_ptr + step_val : step_val;
   const P xc = (P)p[0], yc = (P)p[1], r0 = (P)p[2], beta = (P)p[3], t0 = (P)p[4], bnorm = (P)p[5];
   const P xg = (P)i, yg = (P)j, time = (P)step, hf = (P)0.5, tiny = (P)1e-7, one = (P)1.0;
   P rx, ry, rr, omega, ee0;
@@ -382,7 +382,7 @@ __global__ void k_bc_psr_eval(Geom g, V3<const typename PR::S> e, V3<const typen
   const P ssq = (R * (P)2) * (R * (P)2);
   const P fi = (P)i, fj = (P)j, fk = (P)k;
   if (!((fi - xc) * (fi - xc) + (fj - yc) * (fj - yc) + (fk - zc) * (fk - zc) < ssq)) return;
-  const int step = step_ptr ? *step_ptr : step_val;
+  const int step = step_ptr ? *step_ptr + step_val : step_val;
   const int64_t c = g.at(i, j, k), sy = g.px, sz = g.plane;
 #define A(arr, a, bb, d) ((P)(T)arr[c + (a) + (bb)*sy + (d)*sz])
   const P shift_B = (P)4.0, shift_E = (P)0.0, one = (P)1.0, hf = (P)0.5, qq = (P)0.25;
@@ -579,8 +579,8 @@ __global__ void __launch_bounds__(64) k_reduce_final(const double* part, int nbl
   }
 }
 
-__global__ void k_bump_step(int* step) {
-  if (threadIdx.x == 0) *step += 1;
+__global__ void k_bump_step(int* step, int by) {
+  if (threadIdx.x == 0) *step += by;
 }
 
 // sets a device counter in stream order (a host value copied from the stack
