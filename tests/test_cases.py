@@ -21,6 +21,13 @@ def test_loop_cases_hold_near_null_cells():
         assert n > 0, p.name
 
 
+def test_batch_case_holds_near_null_cells():
+    """So does the scan of the batched-path bitwise test (its loop
+    instances), which covers the batched fix-up kernel."""
+    probs = problems.scan(count=6, nmin=18, nmax=40, seed=3, steps=20)
+    assert sum(int(near_null_cells(p.fields).sum()) for p in probs) > 0
+
+
 def test_near_null_cells_of_a_uniform_field():
     """A uniform B has no near-null cell; zeroing B in a slab makes every
     cell of the slab one (|B|^2 below 1e-30)."""

@@ -387,8 +387,8 @@ int launch_stage_t(tff_handle* h, const StageArgs<typename PR::S, typename PR::P
   return last_launch(h, "k_stage");
 }
 
-// k_stage_fix after a k_stage<PR, TL> launch: one block per CU scans the
-// tiles' plane masks (a few words each) and recomputes what they mark.
+// k_stage_fix after a k_stage<PR, TL> launch: up to one block per CU scans
+// the tiles' plane masks (a few words each) and recomputes what they mark.
 constexpr unsigned FIX_GRID = 256;
 template <class PR, class TL>
 int launch_fix_t(tff_handle* h, const StageArgs<typename PR::S, typename PR::P>& a) {
@@ -400,7 +400,9 @@ int launch_fix_t(tff_handle* h, const StageArgs<typename PR::S, typename PR::P>&
     hipFuncSetAttribute((const void*)k_stage_fix<PR, TL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr = true;
   }
-  hipLaunchKernelGGL((k_stage_fix<PR, TL>), dim3(FIX_GRID), dim3(TL::NT), lds, h->stream, with_tile_grid<TL>(h, a));
+  const auto b = with_tile_grid<TL>(h, a);
+  const unsigned grid = (unsigned)std::min<int64_t>(FIX_GRID, b.nblocks);
+  hipLaunchKernelGGL((k_stage_fix<PR, TL>), dim3(grid), dim3(TL::NT), lds, h->stream, b);
   return last_launch(h, "k_stage_fix");
 }
 
@@ -1172,6 +1174,7 @@ struct tff_batch {
   void* d_info = nullptr;
   void* d_stage = nullptr;  // BatchStage[4][n]: stage 1 from a clean state, stages 1, 2, 3
   int* d_step = nullptr;
+  void* d_fixmask = nullptr;  // FP32-bulk: one word per work-list tile
   hipGraphExec_t graph = nullptr;
 };
 
@@ -1212,6 +1215,19 @@ int batch_launch_stage_t(tff_batch* b, int which) {
                      b->total_tiles);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return bfail(b, TFF_EHIP, std::string("k_batch_stage: ") + hipGetErrorString(e));
+  if constexpr (PR::null_fp64) {
+    static bool fattr = false;
+    if (!fattr) {
+      hipFuncSetAttribute((const void*)k_batch_stage_fix<PR, TL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lds);
+      fattr = true;
+    }
+    const unsigned fgrid = (unsigned)std::min<int64_t>(FIX_GRID, b->total_tiles);
+    hipLaunchKernelGGL((k_batch_stage_fix<PR, TL>), dim3(fgrid), dim3(TL::NT), lds, b->stream, st,
+                       b->d_tile_base, n, b->total_tiles, (const unsigned long long*)b->d_fixmask);
+    e = hipGetLastError();
+    if (e != hipSuccess) return bfail(b, TFF_EHIP, std::string("k_batch_stage_fix: ") + hipGetErrorString(e));
+  }
   return TFF_OK;
 }
 
@@ -1349,12 +1365,20 @@ int batch_build_t(tff_batch* b) {
   hipFree(b->d_floor_base);
   hipFree(b->d_floor_inst);
   hipFree(b->d_info);
+  hipFree(b->d_fixmask);
+  b->d_fixmask = nullptr;
   b->d_floor_base = nullptr;
   b->d_floor_inst = nullptr;
   b->d_stage = b->d_tile_base = nullptr;
   b->d_shell_base = nullptr;
   b->d_shell_inst = nullptr;
   b->d_info = nullptr;
+  if (PR::null_fp64) {
+    BHIP(b, hipMalloc(&b->d_fixmask, (size_t)tt * sizeof(unsigned long long)));
+    for (int w = 0; w < 4; ++w)
+      for (int q = 0; q < n; ++q)
+        st[(size_t)w * n + q].a.fixmask = (unsigned long long*)b->d_fixmask + tile_base[q];
+  }
   BHIP(b, hipMalloc(&b->d_stage, st.size() * sizeof(st[0])));
   BHIP(b, hipMemcpyAsync(b->d_stage, st.data(), st.size() * sizeof(st[0]), hipMemcpyHostToDevice, b->stream));
   BHIP(b, hipMalloc((void**)&b->d_tile_base, n * sizeof(int64_t)));
@@ -1595,6 +1619,7 @@ void tff_batch_destroy(tff_batch* b) {
   hipFree(b->d_floor_inst);
   hipFree(b->d_info);
   hipFree(b->d_stage);
+  hipFree(b->d_fixmask);
   hipFree(b->d_step);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;

@@ -49,7 +49,7 @@ __device__ __forceinline__ int find_item(const int64_t* base, int n, int64_t t) 
 // the grid is a multiple of 8, XCD x works through the x-th contiguous
 // eighth of the list, so tiles that share halo rows share that XCD's L2.
 template <class PR, class TL>
-__global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(sizeof(typename PR::S) == 4 ? 3 : 2)))
+__global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(sizeof(typename PR::S) == 4 ? 4 : 2)))
     k_batch_stage(const BatchStage<typename PR::S, typename PR::P>* __restrict__ st,
                   const int64_t* __restrict__ tile_base, int ninst, int64_t total) {
   extern __shared__ unsigned char smem_raw[];
@@ -64,7 +64,23 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(siz
   for (int64_t t = t0; t < end; t += stride) {
     __syncthreads();  // the previous item has finished with LDS
     const int inst = find_item(tile_base, ninst, t);
-    stage_tile<PR, TL>(st[inst].a, (int)(t - tile_base[inst]), smem_raw);
+    stage_tile<PR, TL, PR::null_fp64>(st[inst].a, (int)(t - tile_base[inst]), smem_raw);
+  }
+}
+
+// The planes a FIX k_batch_stage launch left (FP32-bulk): every instance's
+// tile words are one array, `mask`, in work-list order.
+template <class PR, class TL>
+__global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(2)))
+    k_batch_stage_fix(const BatchStage<typename PR::S, typename PR::P>* __restrict__ st,
+                      const int64_t* __restrict__ tile_base, int ninst, int64_t total,
+                      const unsigned long long* __restrict__ mask) {
+  extern __shared__ unsigned char smem_raw[];
+  for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+    const unsigned long long m = mask[t];
+    if (m == 0ull) continue;
+    const int inst = find_item(tile_base, ninst, t);
+    fix_tile<PR, TL>(st[inst].a, (int)(t - tile_base[inst]), m, smem_raw);
   }
 }
 

@@ -433,27 +433,33 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(siz
   stage_tile<PR, TL, PR::null_fp64>(A, tile_id, smem_raw);
 }
 
-// The planes k_stage skipped (FIX), with the near-null FP64 redo: the same
-// per-cell code as the stage kernel of the unfused path, so the results are
-// those of a kernel that redoes inline.  Block b takes tiles b, b + gridDim.x,
-// ... and recomputes each run of marked planes of each.
+// The planes of tile t that a FIX stage launch marked in `m`, with the
+// near-null FP64 redo: the same per-cell code as the stage kernel of the
+// unfused path, so the results are those of a kernel that redoes inline.
+// Each run of consecutive marked planes is one stage_run.
+template <class PR, class TL>
+__device__ __forceinline__ void fix_tile(const StageArgs<typename PR::S, typename PR::P>& A, int t,
+                                         unsigned long long m, unsigned char* smem_raw) {
+  const int i0 = (t % A.nbx) * TL::TX;
+  const int j0 = ((t / A.nbx) % A.nby) * TL::TY;
+  const int k0 = (t / (A.nbx * A.nby)) * A.kchunk;
+  while (m != 0ull) {
+    const int a = __builtin_ctzll(m);  // first marked plane
+    const unsigned long long rest = ~(m >> a);
+    const int len = rest == 0ull ? 64 - a : __builtin_ctzll(rest);  // marked planes from a on
+    stage_run<PR, TL, false>(A, i0, j0, k0 + a, k0 + a + len, t, smem_raw);
+    m = (a + len >= 64) ? 0ull : (m >> (a + len)) << (a + len);
+  }
+}
+
+// After a FIX k_stage launch: block b takes tiles b, b + gridDim.x, ...
 template <class PR, class TL>
 __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(2)))
     k_stage_fix(StageArgs<typename PR::S, typename PR::P> A) {
   extern __shared__ unsigned char smem_raw[];
   for (int t = (int)blockIdx.x; t < A.nblocks; t += (int)gridDim.x) {
-    unsigned long long m = A.fixmask[t];
-    if (m == 0ull) continue;
-    const int i0 = (t % A.nbx) * TL::TX;
-    const int j0 = ((t / A.nbx) % A.nby) * TL::TY;
-    const int k0 = (t / (A.nbx * A.nby)) * A.kchunk;
-    while (m != 0ull) {
-      const int a = __builtin_ctzll(m);                 // first marked plane
-      const unsigned long long rest = ~(m >> a);
-      const int len = rest == 0ull ? 64 - a : __builtin_ctzll(rest);  // marked planes from a on
-      stage_run<PR, TL, false>(A, i0, j0, k0 + a, k0 + a + len, t, smem_raw);
-      m = (a + len >= 64) ? 0ull : (m >> (a + len)) << (a + len);
-    }
+    const unsigned long long m = A.fixmask[t];
+    if (m != 0ull) fix_tile<PR, TL>(A, t, m, smem_raw);
   }
 }
 
