@@ -45,8 +45,8 @@ def field_errors(get, ref):
 
 def near_null_cells(f):
     """Interior cells whose B interpolant at one of the three E points has
-    |B|^2 below 1e-4 times the cell's own |B|^2 (or below 1e-30): the cells
-    at which the FP32-bulk kernels redo the clean and the currents in FP64.
+    |B|^2 below 1e-4 times the cell's own |B|^2: the cells at which the
+    FP32-bulk kernels redo the clean and the currents in FP64.
     `f` maps bx, by, bz to arrays with ghosts, indexed [k][j][i]."""
     bx, by, bz = (np.asarray(interior(f[c]), np.float64) for c in ("bx", "by", "bz"))
 
@@ -68,5 +68,5 @@ def near_null_cells(f):
     z2 = 0.125 * sum(at(bz, di, dj, dk) for di in (-1, 0) for dj in (-1, 0) for dk in (0, 1))
     near = np.zeros(bx.shape, bool)
     for m in (x0 * x0 + y0 * y0 + z0 * z0, x1 * x1 + y1 * y1 + z1 * z1, x2 * x2 + y2 * y2 + z2 * z2):
-        near |= (m < 1e-4 * bref) | (m < 1e-30)
+        near |= m < 1e-4 * bref
     return near

@@ -29,13 +29,26 @@ def test_batch_case_holds_near_null_cells():
 
 
 def test_near_null_cells_of_a_uniform_field():
-    """A uniform B has no near-null cell; zeroing B in a slab makes every
-    cell of the slab one (|B|^2 below 1e-30)."""
+    """A uniform B has no near-null cell, and neither has a slab of zero or
+    of tiny B (the test is relative to each cell's own |B|^2)."""
     f = {c: np.zeros((12, 12, 12)) for c in ("bx", "by", "bz")}
     f["bz"][...] = 1.0
     assert not near_null_cells(f).any()
     f["bz"][5:7, :, :] = 0.0  # interior planes k = 3, 4
     near = near_null_cells(f)
-    assert near[3:5].all()
-    # the Ez point of plane 2 averages bz over k = 2, 3: still |B| = 1/2
-    assert not near[:2].any() and not near[6:].any()
+    # plane 4: own bz = 0 and the Ez point averages k = 4, 5 (|B| = 1/2);
+    # the Ex point's bz is 0 -> |Bbar|^2 = 0 with bref = 0: exact zero, not near
+    assert not near[4].any()
+    # plane 3: own bz = 0, every interpolant of the Ex/Ey points is 0 and
+    # the Ez point averages k = 3, 4 (zero): all-zero stencil, not near
+    assert not near[3].any()
+    # planes 2 and 5: own bz = 1, the Ez point of plane 2 averages k = 2, 3
+    # (|B| = 1/2): not near
+    assert not near[:3].any() and not near[5:].any()
+    f["bz"][5:7, :, :] = 1e-20
+    assert not near_null_cells(f).any()
+    # bz = +1 on interior plane 4 and -1 on plane 5: plane 4's Ez point
+    # averages the two to zero against its own |B|^2 = 1
+    f["bz"][6, :, :] = 1.0
+    f["bz"][7, :, :] = -1.0
+    assert near_null_cells(f)[4].all()

@@ -72,6 +72,26 @@ def test_multi_step_graphs_keep_the_step(name, precision):
         assert np.array_equal(x, y, equal_nan=True), (f, float(np.nanmax(np.abs(x - y))))
 
 
+@pytest.mark.parametrize("fused", [True, False])
+def test_fp32_bulk_with_tiny_fields(fused):
+    """Every field scaled by 1e-20 (|B|^2 ~ 1e-40, below the smallest normal
+    FP32 number): FP32-bulk's cleanEpar rescales such B interpolants before
+    its hardware reciprocal, so the run stays finite and follows FP64 (the
+    projection does not depend on the scale of B)."""
+    p = problems.alfven(24, seed=2, kmax=4, amp=0.2)
+    for a in p.fields.values():
+        a *= 1e-20
+    ref = _mb(p, "fp64", fused)
+    got = _mb(p, "fp32_bulk", fused)
+    ref.run(0, 6)
+    got.run(0, 6)
+    for f in FIELDS6:
+        assert np.isfinite(got.download(f)).all(), f
+    err = field_errors(got.download, ref.download)
+    print("tiny fields, fused" if fused else "tiny fields, unfused", err)
+    assert max(err.values()) <= 1e-5
+
+
 @pytest.mark.parametrize("name", ["alfven", "loop", "psr"])
 def test_fused_fp64_matches_oracle(name):
     p = CASES[name]()
@@ -91,7 +111,7 @@ def test_fused_fp64_matches_oracle(name):
 def test_results_independent_of_tile_shape(name):
     p = CASES[name]()
     ref = None
-    for tile in range(4):
+    for tile in range(6):
         for kc in (8, 16, 64):
             g = _mb(p, "fp32_bulk", fused=True)
             try:

@@ -114,16 +114,34 @@ __device__ __forceinline__ double clean_point_fp64(const AX& EX, const AY& EY, c
 // |B|^2 == 0 leaves E unchanged.  `near` reports a point whose |B|^2 is
 // below NEAR_NULL times `bref` (when bref > 0).
 template <class T2, class Q2, bool FAST, class AX, class AY, class AZ>
-__device__ __forceinline__ void clean_impl(const AX& EX, const AY& EY, const AZ& EZ, const BBar<Q2>& bb,
+__device__ __forceinline__ void clean_impl(const AX& EX, const AY& EY, const AZ& EZ, const BBar<Q2>& bb_in,
                                            int guard, Q2 bref, Q2& ox, Q2& oy, Q2& oz, bool& near) {
   near = false;
+  // FAST: a point whose |B|^2 is non-zero but below TINY_B2 projects with
+  // its B scaled by 2^64 (exactly; the projection does not depend on the
+  // scale of B), so that the hardware reciprocal stays finite
+  BBar<Q2> bb = bb_in;
+  if constexpr (FAST) {
+    if (fmin(fmin(bb.m[0], bb.m[1]), bb.m[2]) < (Q2)TINY_B2) {
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        if (bb.m[p] > (Q2)0 && bb.m[p] < (Q2)TINY_B2) {
+          const Q2 s = (Q2)0x1p64f;
+          bb.x[p] *= s;
+          bb.y[p] *= s;
+          bb.z[p] *= s;
+          bb.m[p] = sq3<FAST>(bb.x[p], bb.y[p], bb.z[p]);
+        }
+      }
+    }
+  }
   {
     T2 intex, intey, intez;
     e_point<0, T2>(EX, EY, EZ, intex, intey, intez);
     const Q2 bx_ = bb.x[0], by_ = bb.y[0], bz_ = bb.z[0], bmag = bb.m[0];
     Q2 epar = FAST ? mad_<FAST>((Q2)intez, bz_, mad_<FAST>((Q2)intey, by_, (Q2)intex * bx_))
                    : ((Q2)intex * bx_ + (Q2)intey * by_ + (Q2)intez * bz_);
-    near |= bmag < (Q2)NEAR_NULL * bref || (FAST && bmag < (Q2)1e-30f);
+    near |= bb_in.m[0] < (Q2)NEAR_NULL * bref;
     const Q2 e0 = (Q2)EX(0, 0, 0);
     ox = (guard && bmag == (Q2)0) ? e0 : (FAST ? e0 - div_<FAST>(epar * bx_, bmag) : e0 - epar * bx_ / bmag);
   }
@@ -133,7 +151,7 @@ __device__ __forceinline__ void clean_impl(const AX& EX, const AY& EY, const AZ&
     const Q2 bx_ = bb.x[1], by_ = bb.y[1], bz_ = bb.z[1], bmag = bb.m[1];
     Q2 epar = FAST ? mad_<FAST>((Q2)intez, bz_, mad_<FAST>((Q2)intey, by_, (Q2)intex * bx_))
                    : ((Q2)intex * bx_ + (Q2)intey * by_ + (Q2)intez * bz_);
-    near |= bmag < (Q2)NEAR_NULL * bref || (FAST && bmag < (Q2)1e-30f);
+    near |= bb_in.m[1] < (Q2)NEAR_NULL * bref;
     const Q2 e0 = (Q2)EY(0, 0, 0);
     oy = (guard && bmag == (Q2)0) ? e0 : (FAST ? e0 - div_<FAST>(epar * by_, bmag) : e0 - epar * by_ / bmag);
   }
@@ -143,7 +161,7 @@ __device__ __forceinline__ void clean_impl(const AX& EX, const AY& EY, const AZ&
     const Q2 bx_ = bb.x[2], by_ = bb.y[2], bz_ = bb.z[2], bmag = bb.m[2];
     Q2 epar = FAST ? mad_<FAST>((Q2)intez, bz_, mad_<FAST>((Q2)intey, by_, (Q2)intex * bx_))
                    : ((Q2)intex * bx_ + (Q2)intey * by_ + (Q2)intez * bz_);
-    near |= bmag < (Q2)NEAR_NULL * bref || (FAST && bmag < (Q2)1e-30f);
+    near |= bb_in.m[2] < (Q2)NEAR_NULL * bref;
     const Q2 e0 = (Q2)EZ(0, 0, 0);
     oz = (guard && bmag == (Q2)0) ? e0 : (FAST ? e0 - div_<FAST>(epar * bz_, bmag) : e0 - epar * bz_ / bmag);
   }
@@ -156,13 +174,14 @@ __device__ __forceinline__ Q2 cell_b2(const BXa& BXF, const BYa& BYF, const BZa&
   return a * a + b * b + c * c;
 }
 
-// Whether the FP64 redo of clean_cell_bb or rhs_cell_bb can apply at a cell:
-// the union of their two tests over the three E points.
+// Whether the FP64 redo of clean_cell_bb or rhs_cell_bb applies at a cell:
+// an interpolated |B|^2 below NEAR_NULL times the cell's own at one of the
+// three E points (the test both use).
 template <class Q>
 __device__ __forceinline__ bool near_null_cell(const BBar<Q>& bb, Q bref) {
   bool r = false;
 #pragma unroll
-  for (int p = 0; p < 3; ++p) r |= bb.m[p] < (Q)NEAR_NULL * bref || bb.m[p] < (Q)1e-30f;
+  for (int p = 0; p < 3; ++p) r |= bb.m[p] < (Q)NEAR_NULL * bref;
   return r;
 }
 

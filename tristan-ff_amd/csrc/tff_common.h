@@ -11,6 +11,11 @@ constexpr int NG = 2;  // NGHOST, src/CMakeLists.txt:19
 
 // |Bbar|^2 below this fraction of the cell's own |B|^2 counts as near a null
 constexpr float NEAR_NULL = 1e-4f;
+// The fast path's cleanEpar rescales a B interpolant whose |Bbar|^2 is
+// non-zero and below this by 2^64 before the projection: the hardware
+// reciprocal overflows near 1e-38.  (An exact zero keeps the guard's
+// result; a zero that cancels a non-zero B falls under NEAR_NULL.)
+constexpr float TINY_B2 = 1e-30f;
 
 // Device layout of one field component: x fastest, every x-row padded to
 // `px` elements so that the first interior cell of each row is 128-byte
