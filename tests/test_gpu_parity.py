@@ -221,3 +221,7 @@ def test_errors_are_returned_not_raised_in_c():
         g.set_bc(7, ())
     with pytest.raises(pytff.TffError):
         pytff.Meshblock(1)
+    with pytest.raises(pytff.TffError):
+        g.set_tuning(0, 65)  # z-chunks above 64 do not fit a tile's plane mask
+    with pytest.raises(pytff.TffError):
+        g.set_tuning(6, 8)   # no such tile shape

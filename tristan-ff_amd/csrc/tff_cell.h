@@ -122,7 +122,9 @@ __device__ __forceinline__ void clean_impl(const AX& EX, const AY& EY, const AZ&
   // scale of B), so that the hardware reciprocal stays finite
   BBar<Q2> bb = bb_in;
   if constexpr (FAST) {
-    if (fmin(fmin(bb.m[0], bb.m[1]), bb.m[2]) < (Q2)TINY_B2) {
+    // (an all-zero stencil, common where an absorbing layer has zeroed the
+    // fields, stays out of the branch)
+    if (fmin(fmin(bb.m[0], bb.m[1]), bb.m[2]) < (Q2)TINY_B2 && fmax(fmax(bb.m[0], bb.m[1]), bb.m[2]) > (Q2)0) {
 #pragma unroll
       for (int p = 0; p < 3; ++p) {
         if (bb.m[p] > (Q2)0 && bb.m[p] < (Q2)TINY_B2) {
