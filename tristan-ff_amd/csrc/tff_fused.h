@@ -24,6 +24,11 @@
 // currents read | ring slots free.  The per-cell arithmetic is
 // tff_cell.h's, so results equal the unfused kernels bit for bit.
 //
+// FP32-bulk: an output plane whose stencil reads a near-null cell is not
+// computed here but marked for k_stage_fix, which recomputes it with the
+// FP64 redo; the redo's code would otherwise set this kernel's register
+// budget (see stage_run).
+//
 // Addressing: every thread precomputes the in-plane (x-y) element offsets of
 // the cells it loads or writes once per tile; per plane only a uniform base
 // pointer moves, so global accesses take a scalar base plus a 32-bit offset.
