@@ -5,7 +5,7 @@ are also independent of the tile shape and z-chunk the autotuner picks."""
 import numpy as np
 import pytest
 
-from helpers import FIELDS6, field_errors
+from helpers import FIELDS6, field_errors, near_null_cells
 from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -70,6 +70,36 @@ def test_multi_step_graphs_keep_the_step(name, precision):
     for f in FIELDS6:
         x, y = a.download(f, dt), b.download(f, dt)
         assert np.array_equal(x, y, equal_nan=True), (f, float(np.nanmax(np.abs(x - y))))
+
+
+def _nulls_everywhere():
+    """Bz alternating in sign from plane to plane on top of a weak (1e-4)
+    Alfvenic field: the Ez point's average over two planes cancels in every
+    cell, so every plane of every tile takes the near-null FP64 redo."""
+    p = problems.alfven(64, seed=4, kmax=4, amp=1e-4)
+    bz = p.fields["bz"]
+    sign = np.where(np.arange(bz.shape[0]) % 2 == 0, 1.0, -1.0)[:, None, None]
+    bz[2:-2, 2:-2, 2:-2] *= sign[2:-2]
+    return p
+
+
+@pytest.mark.parametrize("kchunk", [8, 64])
+def test_fix_up_of_every_plane(kchunk):
+    """With every plane marked, the stage kernel computes nothing and the
+    fix-up kernel recomputes whole z-chunks (at 64^3 and kchunk 64, a full
+    64-bit plane mask); the result still equals the unfused path bit for
+    bit."""
+    p = _nulls_everywhere()
+    assert near_null_cells(p.fields).mean() > 0.9
+    a = _mb(p, "fp32_bulk", fused=False)
+    b = _mb(p, "fp32_bulk", fused=True)
+    b.set_tuning(3, kchunk)
+    a.run(0, 4)
+    b.run(0, 4)
+    for f in FIELDS6:
+        x, y = a.download(f, np.float32), b.download(f, np.float32)
+        assert np.isfinite(y).all(), f
+        assert np.array_equal(x, y), (f, float(np.max(np.abs(x - y))))
 
 
 @pytest.mark.parametrize("fused", [True, False])
