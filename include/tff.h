@@ -146,6 +146,22 @@ int tff_step(tff_handle* h, int timestep);
  * HIP graph. */
 int tff_run(tff_handle* h, int first_step, int nsteps);
 
+/* The field datasets of writeOutput's flds.tot file for one meshblock at
+ * the origin: writeFields_hdf5 (src/io/write_output.F90:185-429) with
+ * output_istep >= 2, i.e. at nodes 0, istep, 2 istep, ... of each axis, E
+ * interpolated from the edges and B from the faces at offset 0
+ * (interpFromEdges / interpFromFaces, src/tools/helpers.F90:93-257), in the
+ * reference's operation order and the handle's storage precision.
+ * tff_output_dims gives the node counts ceil(n / istep) per axis.
+ * tff_output_fields fills out[0..5] (ex ey ez bx by bz), each
+ * dims[0]*dims[1]*dims[2] values of host_dtype, x fastest (sm_arr(i1, j1,
+ * k1)); it returns after the copies.  The HDF5 file itself (and the var1-3
+ * user datasets and xx/yy/zz coordinates) stay with the host.
+ * output_istep = 1 (the reference's ghost-including debug layout) is
+ * rejected with TFF_EINVAL. */
+int tff_output_dims(tff_handle* h, int istep, int dims[3]);
+int tff_output_fields(tff_handle* h, int istep, void* const out[6], int host_dtype);
+
 /* FP64 diagnostics over the interior (no reference counterpart):
  * out[0] = 1/2 sum E^2, out[1] = 1/2 sum B^2, out[2] = max |div_h B|,
  * out[3] = sqrt(mean (div_h B)^2). */

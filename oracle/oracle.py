@@ -33,6 +33,12 @@ user_loop BC           floor copy by construction (ghost == k=0 plane)
 user_uniform BC        closed-form footpoint field at sample points
 user_psr BC            dipole and tanh shape at sample points; blend limits
                        (s=1 inside -> dipole; s=0 outside -> unchanged)
+output interpolation   bit-for-bit against the reference's own compiled
+(helpers.F90:93-257)   interpFromEdges / interpFromFaces (oracle/_ref), FP64
+                       and FP32, random fields and offsets; linear fields
+                       interpolate exactly at nodes
+output sampling        node count ceil(s / istep) per axis and the nodes
+(write_output.F90)     0, istep, ... against brute force
 =====================  ======================================================
 """
 from __future__ import annotations
@@ -100,6 +106,9 @@ class Oracle:
             lib.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
             lib.orc_reduce.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+            lib.orc_interp.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3 + [ctypes.c_int] * 3 + [ctypes.c_void_p]
+            lib.orc_output_dims.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+            lib.orc_output_fields.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
             Oracle._libs[name] = lib
         self.lib = Oracle._libs[name]
         self.n = (nx, ny, nz)
@@ -170,6 +179,23 @@ class Oracle:
         self.lib.orc_reduce(self.h, out)
         return list(out)
 
+    def interp(self, dx, dy, dz, i, j, k) -> np.ndarray:
+        """interpFromEdges (E) and interpFromFaces (B) at offset (dx, dy, dz)
+        from cell (i, j, k): [ex, ey, ez, bx, by, bz]."""
+        out = np.empty(6, dtype=self.dtype)
+        self.lib.orc_interp(self.h, dx, dy, dz, i, j, k, out.ctypes.data)
+        return out
+
+    def output_fields(self, istep: int) -> dict:
+        """The six node-interpolated field datasets of writeFields_hdf5 for
+        output_istep >= 2, each shaped (k, j, i)."""
+        d = (ctypes.c_int * 3)()
+        self.lib.orc_output_dims(self.h, istep, d)
+        arrs = [np.empty((d[2], d[1], d[0]), dtype=self.dtype) for _ in range(6)]
+        ptrs = (ctypes.c_void_p * 6)(*[a.ctypes.data for a in arrs])
+        self.lib.orc_output_fields(self.h, istep, ptrs)
+        return dict(zip(("ex", "ey", "ez", "bx", "by", "bz"), arrs))
+
 
 class RefSolver:
     """The reference's own m_fldsolver (oracle/_ref), one global instance per
@@ -187,6 +213,7 @@ class RefSolver:
             lib.ref_rk3_step.argtypes = [ctypes.c_double] * 3
             lib.ref_real_size.restype = ctypes.c_int
             lib.ref_run.argtypes = [ctypes.c_int]
+            lib.ref_interp.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int] * 3 + [ctypes.c_void_p]
             RefSolver._libs[name] = lib
         self.lib = RefSolver._libs[name]
         assert self.lib.ref_real_size() == self.dtype.itemsize
@@ -223,3 +250,10 @@ class RefSolver:
     def run(self, nsteps: int):
         """nsteps timesteps of tristanmainloop.F90:50-83 (no user hook)."""
         self.lib.ref_run(nsteps)
+
+    def interp(self, dx, dy, dz, i, j, k) -> np.ndarray:
+        """The reference's own interpFromEdges / interpFromFaces
+        (helpers.F90:93-257): [ex, ey, ez, bx, by, bz] in double."""
+        out = np.empty(6, dtype=np.float64)
+        self.lib.ref_interp(dx, dy, dz, i, j, k, out.ctypes.data)
+        return out

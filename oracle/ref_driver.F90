@@ -3,7 +3,8 @@
 ! reference sources where they lie by oracle/Makefile into oracle/_ref/.
 !
 ! TEST INFRASTRUCTURE ONLY: it lets tests/ check the C oracle against the
-! reference's own initRKstep / rk3Step / cleanEpar, built once with default
+! reference's own initRKstep / rk3Step / cleanEpar and its output
+! interpolation (interpFromEdges / interpFromFaces), built once with default
 ! REAL as shipped (FP32) and once with -fdefault-real-8 -fdefault-double-8.
 ! It sets the module state that src/logistics/initialize.F90:121-149,185-240
 ! would set for a single meshblock and moves arrays in and out; it adds no
@@ -14,6 +15,7 @@ module m_refdriver
   use m_domain
   use m_fields
   use m_fldsolver
+  use m_helpers
   implicit none
 contains
   integer(c_int) function ref_real_size() bind(C, name='ref_real_size')
@@ -144,6 +146,22 @@ contains
       end do
     end do
   end subroutine ref_exchange
+
+  ! The reference's interpFromEdges (E) and interpFromFaces (B) at offset
+  ! (dx, dy, dz) from cell (i, j, k) of the module arrays
+  ! (src/tools/helpers.F90:93-257): out(1:3) = E, out(4:6) = B, widened to
+  ! double (exactly).
+  subroutine ref_interp(dx, dy, dz, i, j, k, out) bind(C, name='ref_interp')
+    real(c_double), value :: dx, dy, dz
+    integer(c_int), value :: i, j, k
+    real(c_double), intent(out) :: out(6)
+    real :: rx, ry, rz, a, b, c
+    rx = dx; ry = dy; rz = dz
+    call interpFromEdges(rx, ry, rz, i, j, k, ex, ey, ez, a, b, c)
+    out(1) = a; out(2) = b; out(3) = c
+    call interpFromFaces(rx, ry, rz, i, j, k, bx, by, bz, a, b, c)
+    out(4) = a; out(5) = b; out(6) = c
+  end subroutine ref_interp
 
   ! nsteps timesteps in the order of src/tristanmainloop.F90:50-83 for a
   ! problem without a user boundary hook (used to time the reference).

@@ -748,3 +748,114 @@ void orc_reduce(const orc_t *o, double *out) {
   out[2] = dmax;
   out[3] = sqrt(d2 / ((double)o->nx * o->ny * o->nz));
 }
+
+/* ------------------------------------------------------------------------
+ * Output path: node interpolation and downsampling.
+ *
+ * Corner averages of interpFromEdges / interpFromFaces
+ * (src/tools/helpers.F90:93-162, :164-257), one row per corner in the order
+ * c000 c100 c010 c110 c001 c101 c011 c111, each row the (di, dj, dk) of its
+ * terms in the reference's term order (the sums round in that order).
+ */
+static const signed char ORC_EDGE[3][8][2][3] = {
+  { /* f_x, :113-126 */
+    {{0, 0, 0}, {-1, 0, 0}}, {{0, 0, 0}, {1, 0, 0}}, {{0, 1, 0}, {-1, 1, 0}}, {{0, 1, 0}, {1, 1, 0}},
+    {{0, 0, 1}, {-1, 0, 1}}, {{0, 0, 1}, {1, 0, 1}}, {{0, 1, 1}, {-1, 1, 1}}, {{0, 1, 1}, {1, 1, 1}} },
+  { /* f_y, :130-143 */
+    {{0, 0, 0}, {0, -1, 0}}, {{1, 0, 0}, {1, -1, 0}}, {{0, 0, 0}, {0, 1, 0}}, {{1, 0, 0}, {1, 1, 0}},
+    {{0, 0, 1}, {0, -1, 1}}, {{1, 0, 1}, {1, -1, 1}}, {{0, 0, 1}, {0, 1, 1}}, {{1, 0, 1}, {1, 1, 1}} },
+  { /* f_z, :147-154 */
+    {{0, 0, 0}, {0, 0, -1}}, {{1, 0, 0}, {1, 0, -1}}, {{0, 1, 0}, {0, 1, -1}}, {{1, 1, 0}, {1, 1, -1}},
+    {{0, 0, 0}, {0, 0, 1}}, {{1, 0, 0}, {1, 0, 1}}, {{0, 1, 0}, {0, 1, 1}}, {{1, 1, 0}, {1, 1, 1}} },
+};
+static const signed char ORC_FACE[3][8][4][3] = {
+  { /* f_x, :184-199 */
+    {{0, 0, 0}, {0, -1, 0}, {0, 0, -1}, {0, -1, -1}},
+    {{1, 0, 0}, {1, -1, 0}, {1, 0, -1}, {1, -1, -1}},
+    {{0, 0, 0}, {0, 1, 0}, {0, 0, -1}, {0, 1, -1}},
+    {{1, 0, 0}, {1, 0, -1}, {1, 1, -1}, {1, 1, 0}},
+    {{0, 0, 0}, {0, 0, 1}, {0, -1, 0}, {0, -1, 1}},
+    {{1, 0, 0}, {1, 0, 1}, {1, -1, 0}, {1, -1, 1}},
+    {{0, 0, 0}, {0, 1, 0}, {0, 1, 1}, {0, 0, 1}},
+    {{1, 0, 0}, {1, 1, 0}, {1, 1, 1}, {1, 0, 1}} },
+  { /* f_y, :209-224 */
+    {{-1, 0, -1}, {-1, 0, 0}, {0, 0, -1}, {0, 0, 0}},
+    {{0, 0, -1}, {0, 0, 0}, {1, 0, -1}, {1, 0, 0}},
+    {{-1, 1, -1}, {-1, 1, 0}, {0, 1, -1}, {0, 1, 0}},
+    {{0, 1, -1}, {0, 1, 0}, {1, 1, -1}, {1, 1, 0}},
+    {{-1, 0, 0}, {-1, 0, 1}, {0, 0, 0}, {0, 0, 1}},
+    {{0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {1, 0, 1}},
+    {{-1, 1, 0}, {-1, 1, 1}, {0, 1, 0}, {0, 1, 1}},
+    {{0, 1, 0}, {0, 1, 1}, {1, 1, 0}, {1, 1, 1}} },
+  { /* f_z, :226-241 */
+    {{-1, -1, 0}, {-1, 0, 0}, {0, -1, 0}, {0, 0, 0}},
+    {{0, -1, 0}, {0, 0, 0}, {1, -1, 0}, {1, 0, 0}},
+    {{-1, 0, 0}, {-1, 1, 0}, {0, 0, 0}, {0, 1, 0}},
+    {{0, 0, 0}, {0, 1, 0}, {1, 0, 0}, {1, 1, 0}},
+    {{-1, -1, 1}, {-1, 0, 1}, {0, -1, 1}, {0, 0, 1}},
+    {{0, -1, 1}, {0, 0, 1}, {1, -1, 1}, {1, 0, 1}},
+    {{-1, 0, 1}, {-1, 1, 1}, {0, 0, 1}, {0, 1, 1}},
+    {{0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}} },
+};
+
+/* the trilinear blend of the eight corner values (every interp* routine:
+ * c00 c10 c01 c11 along x, c0 c1 along y, then z) */
+static real orc_trilinear(const real c[8], real dx, real dy, real dz) {
+  const real c00 = c[0] * (1 - dx) + c[1] * dx;
+  const real c10 = c[2] * (1 - dx) + c[3] * dx;
+  const real c01 = c[4] * (1 - dx) + c[5] * dx;
+  const real c11 = c[6] * (1 - dx) + c[7] * dx;
+  const real c0 = c00 * (1 - dy) + c10 * dy;
+  const real c1 = c01 * (1 - dy) + c11 * dy;
+  return c0 * (1 - dz) + c1 * dz;
+}
+
+/* interpFromEdges (E) and interpFromFaces (B) at offset (dx, dy, dz) from
+ * cell (i, j, k): out[0..2] = E, out[3..5] = B */
+void orc_interp(const orc_t *o, double dx_, double dy_, double dz_, int i, int j, int k, real *out) {
+  const real dx = (real)dx_, dy = (real)dy_, dz = (real)dz_;
+  for (int a = 0; a < 3; ++a) {
+    const real *fe = o->f[F_EX + a], *fb = o->f[F_BX + a];
+    real ce[8], cb[8];
+    for (int c = 0; c < 8; ++c) {
+      const signed char *p = ORC_EDGE[a][c][0], *q = ORC_EDGE[a][c][1];
+      ce[c] = (real)0.5 * (fe[IDX(o, i + p[0], j + p[1], k + p[2])] + fe[IDX(o, i + q[0], j + q[1], k + q[2])]);
+      real s = 0;
+      for (int t = 0; t < 4; ++t) {
+        const signed char *r = ORC_FACE[a][c][t];
+        const real v = fb[IDX(o, i + r[0], j + r[1], k + r[2])];
+        s = t == 0 ? v : s + v;
+      }
+      cb[c] = (real)0.25 * s;
+    }
+    out[a] = orc_trilinear(ce, dx, dy, dz);
+    out[3 + a] = orc_trilinear(cb, dx, dy, dz);
+  }
+}
+
+/* Node count per axis of writeFields_hdf5's downsampled output for
+ * output_istep >= 2 (src/io/write_output.F90:268-293) on one meshblock at
+ * the origin: nodes 0, istep, ... below s, ceil(s / istep) of them. */
+static int orc_out_n(int s, int istep) { return (s + istep - 1) / istep; }
+
+void orc_output_dims(const orc_t *o, int istep, int *dims) {
+  dims[0] = orc_out_n(o->nx, istep);
+  dims[1] = orc_out_n(o->ny, istep);
+  dims[2] = orc_out_n(o->nz, istep);
+}
+
+/* The six field datasets of writeFields_hdf5 (:334-372, the interpolated
+ * branch): out[f] holds dims[0]*dims[1]*dims[2] values, x fastest
+ * (sm_arr(i1, j1, k1)), for f = ex ey ez bx by bz. */
+void orc_output_fields(const orc_t *o, int istep, real **out) {
+  int d[3];
+  orc_output_dims(o, istep, d);
+  for (int k1 = 0; k1 < d[2]; ++k1)
+    for (int j1 = 0; j1 < d[1]; ++j1)
+      for (int i1 = 0; i1 < d[0]; ++i1) {
+        real v[6];
+        orc_interp(o, 0.0, 0.0, 0.0, i1 * istep, j1 * istep, k1 * istep, v);
+        const int64_t t = i1 + (int64_t)d[0] * (j1 + (int64_t)d[1] * k1);
+        for (int f = 0; f < 6; ++f) out[f][t] = v[f];
+      }
+}

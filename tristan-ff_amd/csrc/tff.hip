@@ -19,6 +19,7 @@
 
 #include "tff.h"
 #include "tff_batch.h"
+#include "tff_output.h"
 
 using namespace tff;
 
@@ -707,6 +708,45 @@ struct DoStep {
   }
 };
 
+// The output datasets of writeFields_hdf5 (output_istep >= 2): the cleaned
+// state's node interpolants at every istep-th node, into host buffers.
+int output_dims(const tff_handle* h, int istep, int d[3]) {
+  d[0] = (h->g.nx + istep - 1) / istep;
+  d[1] = (h->g.ny + istep - 1) / istep;
+  d[2] = (h->g.nz + istep - 1) / istep;
+  return TFF_OK;
+}
+
+template <class PR>
+struct DoOutput {
+  static int run(tff_handle* h, int istep, void* const* out, int host_dtype) {
+    using S = typename PR::S;
+    int d[3];
+    output_dims(h, istep, d);
+    const int64_t nout = (int64_t)d[0] * d[1] * d[2];
+    const size_t hsz = host_dtype == TFF_HOST_F64 ? 8 : 4;
+    void* dbuf = nullptr;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (hipMalloc(&dbuf, 6 * (size_t)nout * hsz) != hipSuccess)
+      return fail(h, TFF_ENOMEM, "hipMalloc of the output buffer failed");
+    const dim3 grid((unsigned)((d[0] + 63) / 64), (unsigned)((d[1] + 3) / 4), (unsigned)d[2]);
+    if (hsz == 8)
+      hipLaunchKernelGGL((k_output<S, double>), grid, dim3(256), 0, h->stream, h->g, istep, d[0], d[1], d[2],
+                         cv3<S>(h->es), cv3<S>(h->bs), (double*)dbuf);
+    else
+      hipLaunchKernelGGL((k_output<S, float>), grid, dim3(256), 0, h->stream, h->g, istep, d[0], d[1], d[2],
+                         cv3<S>(h->es), cv3<S>(h->bs), (float*)dbuf);
+    int rc = last_launch(h, "k_output");
+    for (int f = 0; f < 6 && !rc; ++f)
+      if (hipMemcpyAsync(out[f], (char*)dbuf + (size_t)f * nout * hsz, (size_t)nout * hsz, hipMemcpyDeviceToHost,
+                         h->stream) != hipSuccess)
+        rc = fail(h, TFF_EHIP, "hipMemcpyAsync of an output dataset failed");
+    if (hipStreamSynchronize(h->stream) != hipSuccess && !rc) rc = fail(h, TFF_EHIP, "output: stream failed");
+    hipFree(dbuf);
+    return rc;
+  }
+};
+
 template <class PR>
 struct DoRk3 {
   static int run(tff_handle* h, double c1, double c2, double c3) {
@@ -1062,6 +1102,25 @@ int tff_run(tff_handle* h, int first_step, int nsteps) {
     n += done;
   }
   return TFF_OK;
+}
+
+int tff_output_dims(tff_handle* h, int istep, int dims[3]) {
+  if (!h || !dims) return TFF_EINVAL;
+  if (istep < 2)
+    return fail(h, TFF_EINVAL, "output_istep must be >= 2 (the istep = 1 debug layout is not supported)");
+  return output_dims(h, istep, dims);
+}
+
+int tff_output_fields(tff_handle* h, int istep, void* const out[6], int host_dtype) {
+  if (!h || !out) return TFF_EINVAL;
+  if (istep < 2)
+    return fail(h, TFF_EINVAL, "output_istep must be >= 2 (the istep = 1 debug layout is not supported)");
+  if (host_dtype != TFF_HOST_F64 && host_dtype != TFF_HOST_F32) return fail(h, TFF_EINVAL, "unknown host dtype");
+  for (int f = 0; f < 6; ++f)
+    if (!out[f]) return fail(h, TFF_EINVAL, "null output buffer");
+  int rc = ensure_clean(h);
+  if (rc) return rc;
+  return dispatch<DoOutput>(h, istep, out, host_dtype);
 }
 
 int tff_reduce(tff_handle* h, double out[4]) {

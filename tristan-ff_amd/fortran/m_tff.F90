@@ -103,6 +103,20 @@ module m_tff
       real(c_double), intent(out) :: out(4)
     end function tff_reduce
 
+    integer(c_int) function tff_output_dims(h, istep, dims) bind(C, name='tff_output_dims')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: istep
+      integer(c_int), intent(out) :: dims(3)
+    end function tff_output_dims
+
+    integer(c_int) function tff_output_fields(h, istep, out, host_dtype) bind(C, name='tff_output_fields')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: istep, host_dtype
+      type(c_ptr), intent(in) :: out(6)
+    end function tff_output_fields
+
     integer(c_int) function tff_sync(h) bind(C, name='tff_sync')
       import :: c_int, c_ptr
       type(c_ptr), value :: h
