@@ -6,7 +6,9 @@
   copy  -- tff_init_rk_step on the same grid: two launches of a plain copy
            kernel (k_copy3), E -> En and B -> Bn over the allocated box,
            3 * nalloc words each way per launch -- a kernel of known
-           traffic, against which FETCH_SIZE / WRITE_SIZE are calibrated.
+           traffic, against which FETCH_SIZE / WRITE_SIZE are calibrated;
+  output -- tff_output_fields (k_output: writeFields' node interpolation
+           at every istep-th node) on the same workload after two steps.
 
 Each mode prints what it ran (sizes, tile, bytes) so that a counter summary
 can be matched to it.
@@ -27,11 +29,12 @@ import pytff  # noqa: E402
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=("step", "copy"))
+    ap.add_argument("mode", choices=("step", "copy", "output"))
     ap.add_argument("--precision", default="fp32_bulk", choices=("fp64", "fp32", "fp32_bulk"))
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--tile", default="", help="tile:kchunk (default: autotuned)")
+    ap.add_argument("--istep", type=int, default=2, help="output mode: node stride")
     a = ap.parse_args(argv)
     p = problems.alfven(a.n, seed=1, kmax=8)
     mb = pytff.Meshblock(p.n, boundary=p.boundary, cc=p.cc, precision=a.precision)
@@ -47,6 +50,20 @@ def main(argv=None) -> int:
         nalloc = mb.nalloc
         print(f"copy: {a.steps} calls, 2 k_copy3 launches each (E, then B), nalloc {nalloc}, "
               f"{3 * nalloc * word} bytes read and {3 * nalloc * word} written per launch", flush=True)
+        mb.close()
+        return 0
+    if a.mode == "output":
+        mb.run(0, 2)
+        mb.output_fields(a.istep)  # the state is now clean; later calls time the output alone
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            out = mb.output_fields(a.istep)
+        ms = (time.perf_counter() - t0) * 1e3 / max(a.steps, 1)
+        word = 8 if a.precision == "fp64" else 4
+        nout = out["ex"].size
+        print(f"output: {a.precision} {a.n}^3, istep {a.istep}, {nout} nodes x 6 datasets, "
+              f"{6 * p.cells * word} bytes of fields read (each cell once) and {6 * nout * 4} written "
+              f"per k_output launch; {ms:.3f} ms per call including the copies to the host", flush=True)
         mb.close()
         return 0
     if a.tile:
