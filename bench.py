@@ -153,8 +153,8 @@ def main(argv=None) -> int:
                          "the same command time the same kernel; auto: the autotuner")
     a = ap.parse_args(argv)
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
-              "multi-GPU sharding of the meshblock is a design note, not a measured path.",
+        print(f"bench.py: the force-free step runs on one GPU only (asked for --gpus {a.gpus}); "
+              "sharding the box over several GPUs is designed but not built or measured.",
               file=sys.stderr)
         return 2
     import problems
