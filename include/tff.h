@@ -12,7 +12,8 @@
  *    aborts.  tff_last_error() gives the text of the last failure.
  *  - A handle owns all device memory of one meshblock and one HIP stream.
  *    Calls are asynchronous on that stream except tff_upload, tff_download,
- *    tff_reduce and tff_sync, which return after the stream has drained.
+ *    tff_output_fields, tff_reduce and tff_sync, which return after the
+ *    stream has drained.
  *  - A handle is not thread-safe: call it from one host thread.
  *  - Host arrays passed to tff_upload / tff_download hold one field
  *    component in the reference's layout: Fortran order (x fastest),
