@@ -511,8 +511,15 @@ int ghost_passes(tff_handle* h, const Vec& ge_in, const Vec& prev_b, const Vec& 
   const int64_t n = shell_count(h);
   const int tb = 256;
   const unsigned nb = (unsigned)((n + tb - 1) / tb);
+  // a hook that reads no field (none, the uniform twist, the driven loop)
+  // cannot see a carried value that the periodic fill then overwrites
+  const int* bd = h->cfg.boundary;
+  const bool blind = h->bc_kind == TFF_BC_NONE || h->bc_kind == TFF_BC_UNIFORM ||
+                     (h->bc_kind == TFF_BC_LOOP && h->bc_host[8] == 2.0);
+  const int3 per = blind ? int3{bd[0] == TFF_PERIODIC, bd[1] == TFF_PERIODIC, bd[2] == TFF_PERIODIC}
+                         : int3{0, 0, 0};
   hipLaunchKernelGGL(k_carry<S>, dim3(nb), dim3(tb), 0, h->stream, h->g, cv3<S>(ge_in), cv3<S>(prev_b),
-                     mv3<S>(oe), mv3<S>(ob), n);
+                     mv3<S>(oe), mv3<S>(ob), n, per);
   int rc = last_launch(h, "k_carry");
   if (rc) return rc;
   if ((rc = launch_bc<PR>(h, oe, ob, step_ptr, step_val))) return rc;
@@ -596,7 +603,8 @@ int ensure_clean_t(tff_handle* h) {
     const int64_t n = shell_count(h);
     const int tb = 256;
     hipLaunchKernelGGL(k_carry<S>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, h->stream, h->g,
-                       cv3<S>(h->ge), cv3<S>(h->bs), mv3<S>(h->ce), V3<S>{nullptr, nullptr, nullptr}, n);
+                       cv3<S>(h->ge), cv3<S>(h->bs), mv3<S>(h->ce), V3<S>{nullptr, nullptr, nullptr}, n,
+                       int3{0, 0, 0});
     if ((rc = last_launch(h, "k_carry"))) return rc;
   }
   std::swap(h->es, h->ce);

@@ -473,13 +473,16 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(2))
 
 // The ghost cells of a new raw stage output start as the previous stage's
 // cleaned fields there (rk3Update leaves ghosts alone): E from `ge`, B from
-// the previous raw B.
+// the previous raw B.  With `per` (the periodic axes) set, a ghost cell that
+// the periodic fill will overwrite before anything reads it is skipped.
 template <class S>
-__global__ void k_carry(Geom g, V3<const S> ge, V3<const S> pb, V3<S> oe, V3<S> ob, int64_t nshell) {
+__global__ void k_carry(Geom g, V3<const S> ge, V3<const S> pb, V3<S> oe, V3<S> ob, int64_t nshell, int3 per) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nshell) return;
   int i, j, k;
   shell_cell(g, t, i, j, k);
+  const bool ox = i < 0 || i >= g.nx, oy = j < 0 || j >= g.ny, oz = k < 0 || k >= g.nz;
+  if (!((ox && !per.x) || (oy && !per.y) || (oz && !per.z))) return;
   const int64_t c = g.at(i, j, k);
   oe.x[c] = ge.x[c];
   oe.y[c] = ge.y[c];
