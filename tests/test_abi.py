@@ -12,7 +12,8 @@ def test_header_declares_the_reference_procedures():
     names = set(pytff.header_functions())
     for n in ("tff_init_rk_step", "tff_rk3_step", "tff_clean_epar", "tff_exchange_fields",
               "tff_user_bc", "tff_step", "tff_run", "tff_create", "tff_destroy", "tff_upload",
-              "tff_download", "tff_reduce", "tff_set_bc", "tff_last_error"):
+              "tff_download", "tff_reduce", "tff_set_bc", "tff_last_error",
+              "tff_output_dims", "tff_output_fields"):
         assert n in names, n
 
 
