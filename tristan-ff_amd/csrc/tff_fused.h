@@ -29,9 +29,9 @@
 // FP64 redo; the redo's code would otherwise set this kernel's register
 // budget (see stage_run).
 //
-// Addressing: every thread precomputes the in-plane (x-y) element offsets of
+// Addressing: every thread precomputes the in-plane (x-y) byte offsets of
 // the cells it loads or writes once per tile; per plane only a uniform base
-// pointer moves, so global accesses take a scalar base plus a 32-bit offset.
+// pointer moves (each access is that base plus the lane's 32-bit offset).
 //
 // Ghost cells: with every axis periodic (`wrap`), halo reads wrap around the
 // box and no ghost cell is read or written.  Otherwise ghosts are stored:
