@@ -29,3 +29,32 @@ def test_config_struct_matches_header_layout():
     # nx, ny, nz, boundary[3], absorb_buff, absorb_fixed, precision, device (10 ints), cc
     assert ctypes.sizeof(pytff.Config) == 10 * 4 + 8
     assert pytff.Config.cc.offset == 40
+
+
+def test_no_cpu_fallback_without_the_library():
+    """With the library missing, the Python host raises instead of
+    computing anything on the CPU."""
+    import subprocess
+    import sys
+    env = dict(os.environ, TFF_LIBRARY="/nonexistent/libtff.so")
+    code = ("import pytff\n"
+            "try:\n"
+            "    pytff.Meshblock(8)\n"
+            "except pytff.TffError as e:\n"
+            "    print('raised:', e)\n")
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120,
+                       cwd=os.path.dirname(pytff.__file__))
+    assert "raised:" in r.stdout and "not built" in r.stdout, (r.stdout, r.stderr)
+
+
+def test_no_cpu_fallback_without_a_gpu():
+    """On a machine without a GPU, creating a meshblock fails with an error
+    from the C ABI rather than running elsewhere."""
+    if not os.path.exists(pytff.LIB_PATH):
+        pytest.skip("libtff.so not built")
+    try:
+        g = pytff.Meshblock(8)
+    except pytff.TffError:
+        return
+    g.close()
+    pytest.skip("a GPU is present: the CPU-only behaviour is not observable here")
