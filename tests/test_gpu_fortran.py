@@ -127,3 +127,23 @@ def test_fortran_loop_fused_equals_reference_order(tmp_path):
     err = field_errors(lambda f: outs[0][f], o.field)
     print("fortran loop", err)
     assert max(err.values()) <= 1e-12
+
+
+def test_fortran_output_datasets_match_oracle(tmp_path):
+    """With <output> istep = 2 the driver writes the six field datasets of
+    writeFields_hdf5 from tff_output_fields; they equal the oracle's
+    interpolation of the same dumped state (FP64, rounded to FP32 on output)."""
+    n, last = 24, 10
+    deck = EMWAVE_DECK.format(n=n, last=last, interval=5, fused=1).replace(
+        "dump          = 1", "dump          = 1\n  istep         = 2")
+    out, _ = _run("tristan_ff_emwave", deck, tmp_path, "istep2")
+    state = _read_dump(out / f"flds.{last:05d}.bin", n)
+    o = Oracle(n, boundary=(1, 1, 1), cc=0.5)
+    o.set_fields(**state)
+    want = o.output_fields(2)
+    m = n // 2
+    got = np.fromfile(out / f"flds.tot.{last:05d}.bin", dtype=np.float32)
+    assert got.size == 6 * m ** 3
+    got = got.reshape(6, m, m, m)
+    for q, f in enumerate(("ex", "ey", "ez", "bx", "by", "bz")):
+        assert np.array_equal(got[q], want[f].astype(np.float32)), f

@@ -16,6 +16,9 @@ module m_globalnamespace
   real(dprec) :: CC, CCINV
   integer :: final_timestep
   integer :: output_start = 0, output_interval = 10, output_dump = 0
+  ! <output> istep (initialize.F90:179): node stride of the flds.tot datasets;
+  ! below 2 (this driver's default) they are not written
+  integer :: output_istep = 0
   integer :: precision_mode = 0, fused_step = 1
   character(len=STR_MAX) :: input_file_name = 'input', output_dir_name = 'output'
 end module m_globalnamespace

@@ -40,6 +40,7 @@ contains
     call getInput('output', 'start', output_start, 0)
     call getInput('output', 'interval', output_interval, 10)
     call getInput('output', 'dump', output_dump, 0)
+    call getInput('output', 'istep', output_istep, 0)
     call getInput('time', 'last', final_timestep, 1000)
     call getInput('algorithm', 'c', CC, 0.45d0)
     call getInput('algorithm', 'precision', precision_mode, 0)
@@ -135,14 +136,20 @@ contains
     end do
   end subroutine referenceSteps
 
-  ! Energy and div B history (FP64 reductions on the device) and, with
-  ! <output> dump = 1, the fields with ghosts as raw FP64 (x fastest).
+  ! Energy and div B history (FP64 reductions on the device); with
+  ! <output> dump = 1, the fields with ghosts as raw FP64 (x fastest); with
+  ! <output> istep >= 2, the six field datasets of writeFields_hdf5
+  ! (write_output.F90:334-374: ex ey ez bx by bz node interpolants at every
+  ! istep-th node) as raw FP32, one after the other, x fastest.
   subroutine writeOutput(step, ms)
     integer, intent(in) :: step
     real(dprec), intent(in) :: ms
     real(c_double) :: d(4)
     character(len=STR_MAX) :: fname
-    integer :: u
+    integer :: u, f
+    integer(c_int) :: dims(3)
+    real(c_float), allocatable, target :: sm(:, :, :, :)
+    type(c_ptr) :: bufs(6)
     call tffCheck(tff_reduce(tff_h, d), 'tff_reduce')
     if (step < 0) then
       write (*, '(A,ES24.16,A,ES24.16,A,ES10.3)') 'init          E2/2', d(1), '  B2/2', d(2), &
@@ -158,6 +165,20 @@ contains
       open (newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
       write (u) ex, ey, ez, bx, by, bz
       close (u)
+    end if
+    if ((output_istep >= 2) .and. (step >= 0)) then
+      call tffCheck(tff_output_dims(tff_h, output_istep, dims), 'tff_output_dims')
+      allocate (sm(dims(1), dims(2), dims(3), 6))
+      do f = 1, 6
+        bufs(f) = c_loc(sm(1, 1, 1, f))
+      end do
+      call tffCheck(tff_output_fields(tff_h, output_istep, bufs, TFF_HOST_F32), 'tff_output_fields')
+      write (fname, '(A,A,I5.5,A)') trim(output_dir_name), '/flds.tot.', step, '.bin'
+      call execute_command_line('mkdir -p '//trim(output_dir_name))
+      open (newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
+      write (u) sm
+      close (u)
+      deallocate (sm)
     end if
   end subroutine writeOutput
 end module m_mainloop
