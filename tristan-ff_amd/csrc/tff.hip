@@ -362,14 +362,23 @@ R with_tile(int t, F&& f) {
 constexpr size_t CU_LDS = 160 * 1024;
 template <class PR, class TL>
 constexpr bool tile_fits() {
-  return TL::template lds_bytes<typename PR::S, typename PR::T>() <= CU_LDS;
+  return stage_lds_max<PR, TL>() <= CU_LDS;
 }
 
 template <class PR>
 size_t tile_lds(int t) {
-  using S = typename PR::S;
-  using T = typename PR::T;
-  return with_tile<size_t>(t, [](auto tl) { return decltype(tl)::template lds_bytes<S, T>(); });
+  return with_tile<size_t>(t, [](auto tl) { return stage_lds_max<PR, decltype(tl)>(); });
+}
+
+// launch kernel `k` (a stage-kernel instance) with `lds` bytes of dynamic
+// LDS, raising its LDS limit on the first launch
+template <class K, class A>
+void launch_lds(K k, bool& attr, unsigned grid, unsigned block, size_t lds, hipStream_t st, const A& a) {
+  if (!attr) {
+    hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = true;
+  }
+  hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, st, a);
 }
 
 bool all_periodic(const tff_handle* h) {
@@ -396,21 +405,18 @@ SA with_tile_grid(const tff_handle* h, const SA& a) {
 
 template <class PR, class TL>
 int launch_stage_t(tff_handle* h, const StageArgs<typename PR::S, typename PR::P>& a) {
-  using S = typename PR::S;
-  using T = typename PR::T;
   if constexpr (!tile_fits<PR, TL>()) return fail(h, TFF_EINVAL, "tile shape exceeds the LDS of a CU");
   else {
-  const size_t lds = TL::template lds_bytes<S, T>();
-  static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void*)k_stage<PR, TL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr = true;
-  }
+  const size_t lds = stage_lds<PR, TL, PR::null_fp64>();
+  static bool attr_p = false, attr_s = false;
   const auto b = with_tile_grid<TL>(h, a);
   if (PR::null_fp64 && (!b.fixmask || b.nblocks > h->fix_cap))
     return fail(h, TFF_ESTATE, "near-null plane masks missing or too small");
   const unsigned grid = (unsigned)((b.nblocks + 7) / 8 * 8);
-  hipLaunchKernelGGL((k_stage<PR, TL>), dim3(grid), dim3(TL::NT), lds, h->stream, b);
+  if (b.wrap)
+    launch_lds(k_stage<PR, TL, WRAP_PERIODIC>, attr_p, grid, TL::NT, lds, h->stream, b);
+  else
+    launch_lds(k_stage<PR, TL, WRAP_STORED>, attr_s, grid, TL::NT, lds, h->stream, b);
   return last_launch(h, "k_stage");
   }
 }
@@ -420,17 +426,14 @@ int launch_stage_t(tff_handle* h, const StageArgs<typename PR::S, typename PR::P
 constexpr unsigned FIX_GRID = 256;
 template <class PR, class TL>
 int launch_fix_t(tff_handle* h, const StageArgs<typename PR::S, typename PR::P>& a) {
-  using S = typename PR::S;
-  using T = typename PR::T;
-  const size_t lds = TL::template lds_bytes<S, T>();
-  static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void*)k_stage_fix<PR, TL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr = true;
-  }
+  const size_t lds = stage_lds<PR, TL, false>();
+  static bool attr_p = false, attr_s = false;
   const auto b = with_tile_grid<TL>(h, a);
   const unsigned grid = (unsigned)std::min<int64_t>(FIX_GRID, b.nblocks);
-  hipLaunchKernelGGL((k_stage_fix<PR, TL>), dim3(grid), dim3(TL::NT), lds, h->stream, b);
+  if (b.wrap)
+    launch_lds(k_stage_fix<PR, TL, WRAP_PERIODIC>, attr_p, grid, TL::NT, lds, h->stream, b);
+  else
+    launch_lds(k_stage_fix<PR, TL, WRAP_STORED>, attr_s, grid, TL::NT, lds, h->stream, b);
   return last_launch(h, "k_stage_fix");
 }
 
@@ -1308,7 +1311,7 @@ int batch_launch_stage_t(tff_batch* b, int which) {
   using P = typename PR::P;
   if constexpr (!tile_fits<PR, TL>()) return bfail(b, TFF_EINVAL, "tile shape exceeds the LDS of a CU");
   else {
-  const size_t lds = TL::template lds_bytes<S, T>();
+  const size_t lds = stage_lds<PR, TL, PR::null_fp64>();
   static bool attr = false;
   if (!attr) {
     hipFuncSetAttribute((const void*)k_batch_stage<PR, TL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1322,14 +1325,15 @@ int batch_launch_stage_t(tff_batch* b, int which) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return bfail(b, TFF_EHIP, std::string("k_batch_stage: ") + hipGetErrorString(e));
   if constexpr (PR::null_fp64) {
+    const size_t flds = stage_lds<PR, TL, false>();
     static bool fattr = false;
     if (!fattr) {
       hipFuncSetAttribute((const void*)k_batch_stage_fix<PR, TL>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)lds);
+                          (int)flds);
       fattr = true;
     }
     const unsigned fgrid = (unsigned)std::min<int64_t>(FIX_GRID, b->total_tiles);
-    hipLaunchKernelGGL((k_batch_stage_fix<PR, TL>), dim3(fgrid), dim3(TL::NT), lds, b->stream, st,
+    hipLaunchKernelGGL((k_batch_stage_fix<PR, TL>), dim3(fgrid), dim3(TL::NT), flds, b->stream, st,
                        b->d_tile_base, n, b->total_tiles, (const unsigned long long*)b->d_fixmask);
     e = hipGetLastError();
     if (e != hipSuccess) return bfail(b, TFF_EHIP, std::string("k_batch_stage_fix: ") + hipGetErrorString(e));
@@ -1349,7 +1353,7 @@ int batch_grid_t(tff_batch* b) {
   using T = typename PR::T;
   if constexpr (!tile_fits<PR, TL>()) return bfail(b, TFF_EINVAL, "tile shape exceeds the LDS of a CU");
   else {
-  const size_t lds = TL::template lds_bytes<S, T>();
+  const size_t lds = stage_lds<PR, TL, PR::null_fp64>();
   hipFuncSetAttribute((const void*)k_batch_stage<PR, TL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   int per_cu = 0;
   BHIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_batch_stage<PR, TL>, TL::NT, lds));

@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(siz
   for (int64_t t = t0; t < end; t += stride) {
     __syncthreads();  // the previous item has finished with LDS
     const int inst = find_item(tile_base, ninst, t);
-    stage_tile<PR, TL, PR::null_fp64>(st[inst].a, (int)(t - tile_base[inst]), smem_raw);
+    stage_tile<PR, TL, PR::null_fp64, WRAP_RUNTIME>(st[inst].a, (int)(t - tile_base[inst]), smem_raw);
   }
 }
 
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(2))
     const unsigned long long m = mask[t];
     if (m == 0ull) continue;
     const int inst = find_item(tile_base, ninst, t);
-    fix_tile<PR, TL>(st[inst].a, (int)(t - tile_base[inst]), m, smem_raw);
+    fix_tile<PR, TL, WRAP_RUNTIME>(st[inst].a, (int)(t - tile_base[inst]), m, smem_raw);
   }
 }
 

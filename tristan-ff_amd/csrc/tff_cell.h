@@ -293,6 +293,70 @@ __device__ __forceinline__ double current_point_fp64(const AX& EX, const AY& EY,
   return cc * (intrho * cross) * rcp_nr(bmag + 1e-7);  // TINY, src/defs.F90:4
 }
 
+// The drift currents of rk3Push at one cell given the B interpolants `bb`
+// and raw |B|^2 `bref` of the cell.  RHO(a, b, d) gives the charge density
+// at node (i+a, j+b, k+d).  REDO: a near-null cell redoes them in FP64 from
+// the raw B (BXF, BYF, BZF are read only then).
+template <class PR, bool REDO, class AX, class AY, class AZ, class BXa, class BYa, class BZa, class RF>
+__device__ __forceinline__ void currents_of(const AX& EX, const AY& EY, const AZ& EZ, const BXa& BXF,
+                                            const BYa& BYF, const BZa& BZF, const RF& RHO,
+                                            const BBar<typename PR::Q>& bb, typename PR::Q bref,
+                                            typename PR::P cc, typename PR::P& jx, typename PR::P& jy,
+                                            typename PR::P& jz) {
+  using T = typename PR::T;
+  using Q = typename PR::Q;
+  using P = typename PR::P;
+  const T r000 = RHO(0, 0, 0);
+  const T r100 = RHO(1, 0, 0);
+  const T r010 = RHO(0, 1, 0);
+  const T r001 = RHO(0, 0, 1);
+  Q j[3];
+  bool near;
+  currents_impl<T, Q, PR::null_fp64>(EX, EY, EZ, bb, (Q)cc, r000, r100, r010, r001, bref, j, near);
+  if constexpr (REDO) {
+    if (near) {
+      // FP64 redo, one current at a time (small register footprint)
+      const double c = (double)cc, h = 0.5;
+      const double q000 = (double)r000;
+      jx = (P)current_point_fp64<0>(EX, EY, EZ, BXF, BYF, BZF, c, h * ((double)r100 + q000));
+      jy = (P)current_point_fp64<1>(EX, EY, EZ, BXF, BYF, BZF, c, h * ((double)r010 + q000));
+      jz = (P)current_point_fp64<2>(EX, EY, EZ, BXF, BYF, BZF, c, h * (q000 + (double)r001));
+      return;
+    }
+  }
+  jx = (P)j[0];
+  jy = (P)j[1];
+  jz = (P)j[2];
+}
+
+// The raw-B curl brackets of rk3Push at a cell (field_solver.F90:84-89):
+// cb[0] = (by(k) - by(k-1)) - (bz(j) - bz(j-1)), and cyclically.
+template <class T, class BXa, class BYa, class BZa>
+__device__ __forceinline__ void curl_b(const BXa& BXF, const BYa& BYF, const BZa& BZF, T* cb) {
+  cb[0] = (BYF(0, 0, 0) - BYF(0, 0, -1)) - (BZF(0, 0, 0) - BZF(0, -1, 0));
+  cb[1] = (BZF(0, 0, 0) - BZF(-1, 0, 0)) - (BXF(0, 0, 0) - BXF(0, 0, -1));
+  cb[2] = (BXF(0, 0, 0) - BXF(0, -1, 0)) - (BYF(0, 0, 0) - BYF(-1, 0, 0));
+}
+
+// dE and dB of rk3Push (field_solver.F90:81-89, :160-162) from the
+// currents, the raw-B curl brackets cb and the B0 curl bracket t:
+// d[0..2] = dE, d[3..5] = dB.
+template <class PR, class AX, class AY, class AZ>
+__device__ __forceinline__ void rhs_tail(const AX& EX, const AY& EY, const AZ& EZ, typename PR::P cc,
+                                         typename PR::T tx, typename PR::T ty, typename PR::T tz,
+                                         const typename PR::T* cb, typename PR::P jx, typename PR::P jy,
+                                         typename PR::P jz, typename PR::P* d) {
+  using P = typename PR::P;
+  const P dbx = cc * (P)((EY(0, 0, 1) - EY(0, 0, 0)) - (EZ(0, 1, 0) - EZ(0, 0, 0)));
+  const P dby = cc * (P)((EZ(1, 0, 0) - EZ(0, 0, 0)) - (EX(0, 0, 1) - EX(0, 0, 0)));
+  const P dbz = cc * (P)((EX(0, 1, 0) - EX(0, 0, 0)) - (EY(1, 0, 0) - EY(0, 0, 0)));
+  const P dex = -cc * (P)(cb[0] - tx) - jx;
+  const P dey = -cc * (P)(cb[1] - ty) - jy;
+  const P dez = -cc * (P)(cb[2] - tz) - jz;
+  d[0] = dex; d[1] = dey; d[2] = dez;
+  d[3] = dbx; d[4] = dby; d[5] = dbz;
+}
+
 // rk3Push at one cell (field_solver.F90:81-162) given the B interpolants
 // `bb` and raw |B|^2 `bref` of the cell: d[0..2] = dE, d[3..5] = dB.
 // RHO(a, b, d) gives the charge density at node (i+a, j+b, k+d); t0 is the
@@ -305,38 +369,29 @@ __device__ __forceinline__ void rhs_cell_bb(const AX& EX, const AY& EY, const AZ
                                             typename PR::P cc, typename PR::T tx, typename PR::T ty,
                                             typename PR::T tz, typename PR::P* d) {
   using T = typename PR::T;
-  using Q = typename PR::Q;
   using P = typename PR::P;
   // the currents first: their FP64 redo then runs with the fewest other
   // values live
-  const T r000 = RHO(0, 0, 0);
-  const T r100 = RHO(1, 0, 0);
-  const T r010 = RHO(0, 1, 0);
-  const T r001 = RHO(0, 0, 1);
-  Q j[3];
-  bool near;
-  currents_impl<T, Q, PR::null_fp64>(EX, EY, EZ, bb, (Q)cc, r000, r100, r010, r001, bref, j, near);
   P jx, jy, jz;
-  if (REDO && near) {
-    // FP64 redo, one current at a time (small register footprint)
-    const double c = (double)cc, h = 0.5;
-    const double q000 = (double)r000;
-    jx = (P)current_point_fp64<0>(EX, EY, EZ, BXF, BYF, BZF, c, h * ((double)r100 + q000));
-    jy = (P)current_point_fp64<1>(EX, EY, EZ, BXF, BYF, BZF, c, h * ((double)r010 + q000));
-    jz = (P)current_point_fp64<2>(EX, EY, EZ, BXF, BYF, BZF, c, h * (q000 + (double)r001));
-  } else {
-    jx = (P)j[0];
-    jy = (P)j[1];
-    jz = (P)j[2];
-  }
-  const P dbx = cc * (P)((EY(0, 0, 1) - EY(0, 0, 0)) - (EZ(0, 1, 0) - EZ(0, 0, 0)));
-  const P dby = cc * (P)((EZ(1, 0, 0) - EZ(0, 0, 0)) - (EX(0, 0, 1) - EX(0, 0, 0)));
-  const P dbz = cc * (P)((EX(0, 1, 0) - EX(0, 0, 0)) - (EY(1, 0, 0) - EY(0, 0, 0)));
-  const P dex = -cc * (P)(((BYF(0, 0, 0) - BYF(0, 0, -1)) - (BZF(0, 0, 0) - BZF(0, -1, 0))) - tx) - jx;
-  const P dey = -cc * (P)(((BZF(0, 0, 0) - BZF(-1, 0, 0)) - (BXF(0, 0, 0) - BXF(0, 0, -1))) - ty) - jy;
-  const P dez = -cc * (P)(((BXF(0, 0, 0) - BXF(0, -1, 0)) - (BYF(0, 0, 0) - BYF(-1, 0, 0))) - tz) - jz;
-  d[0] = dex; d[1] = dey; d[2] = dez;
-  d[3] = dbx; d[4] = dby; d[5] = dbz;
+  currents_of<PR, REDO>(EX, EY, EZ, BXF, BYF, BZF, RHO, bb, bref, cc, jx, jy, jz);
+  T cb[3];
+  curl_b<T>(BXF, BYF, BZF, cb);
+  rhs_tail<PR>(EX, EY, EZ, cc, tx, ty, tz, cb, jx, jy, jz, d);
+}
+
+// rk3Push at one cell from precomputed raw-B terms: the B interpolants, the
+// raw |B|^2 and the curl brackets of the cell's raw B (no near-null redo).
+template <class PR, class AX, class AY, class AZ, class RF>
+__device__ __forceinline__ void rhs_cell_pre(const AX& EX, const AY& EY, const AZ& EZ, const RF& RHO,
+                                             const BBar<typename PR::Q>& bb, typename PR::Q bref,
+                                             typename PR::P cc, typename PR::T tx, typename PR::T ty,
+                                             typename PR::T tz, const typename PR::T* cb, typename PR::P* d) {
+  using T = typename PR::T;
+  using P = typename PR::P;
+  auto none = [](int, int, int) { return (T)0; };
+  P jx, jy, jz;
+  currents_of<PR, false>(EX, EY, EZ, none, none, none, RHO, bb, bref, cc, jx, jy, jz);
+  rhs_tail<PR>(EX, EY, EZ, cc, tx, ty, tz, cb, jx, jy, jz, d);
 }
 
 template <class PR, class AX, class AY, class AZ, class BXa, class BYa, class BZa, class RF>

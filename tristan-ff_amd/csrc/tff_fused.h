@@ -49,11 +49,45 @@ struct Tile {
   static constexpr int W1 = TX + 2, H1 = TY + 2, N1 = W1 * H1;
   static constexpr int NT = TX * TY;
   static constexpr int L2PT = (N2 + NT - 1) / NT;
-  template <class S, class T>
-  static constexpr size_t lds_bytes() {
-    return sizeof(S) * (size_t)(18 * N2 + 7 * N1) + 4 * sizeof(unsigned);  // rings + near-null flags
+};
+
+// Plane rings of stage_run in LDS: how many planes of each field it keeps.
+//
+// The clean of plane q reads raw ex, ey at q..q+1, ez at q-1..q, bx, by at
+// q-1..q and bz at q..q+1; the push of plane k reads the cleaned E at
+// k-1..k+1.  TWO (FP32 policies without the inline FP64 redo): the push
+// also takes its raw-B terms (the curl brackets and the cell's own B) from
+// registers that the own-cell clean filled one plane earlier, so it reads
+// no raw plane at all.  Raw plane k+3 may then be stored while other waves
+// still push plane k, and a plane needs two barriers.  Otherwise the push
+// reads raw B at k-1..k+1, and a third barrier protects the rings.
+template <class PR, bool FIX>
+struct Ring {
+  static constexpr bool REUSE = sizeof(typename PR::Q) == 4;
+  static constexpr bool TWO = REUSE && !(PR::null_fp64 && !FIX);
+  static constexpr int EX = 2, EY = 2, EZ = 3;
+  static constexpr int BX = TWO ? 3 : 4, BY = TWO ? 3 : 4, BZ = TWO ? 2 : 3;
+  static constexpr int RAW = EX + EY + EZ + BX + BY + BZ;
+  static constexpr int CLEAN = 2 + 2 + 3;
+  // near-null flags of the planes (FP32-bulk), one word per plane slot
+  static constexpr int FLAGS = TWO ? 8 : 4;
+  template <class TL>
+  static constexpr size_t bytes() {
+    return sizeof(typename PR::S) * (size_t)(RAW * TL::N2 + CLEAN * TL::N1) + FLAGS * sizeof(unsigned);
   }
 };
+
+// LDS of a stage_run<PR, TL, FIX> workgroup
+template <class PR, class TL, bool FIX>
+constexpr size_t stage_lds() {
+  return Ring<PR, FIX>::template bytes<TL>();
+}
+// the larger of the stage kernel's and its fix-up kernel's LDS
+template <class PR, class TL>
+constexpr size_t stage_lds_max() {
+  return stage_lds<PR, TL, false>() > stage_lds<PR, TL, PR::null_fp64>() ? stage_lds<PR, TL, false>()
+                                                                       : stage_lds<PR, TL, PR::null_fp64>();
+}
 
 // Field buffers are passed as one base pointer each: component c of a
 // buffer starts g.cs elements after component c-1 (fewer kernel arguments
@@ -87,12 +121,19 @@ __device__ __forceinline__ int wrap_index(int v, int n) {
 
 // element at a uniform base plus a 32-bit byte offset (the form global
 // memory instructions take as scalar base + vector offset)
+//
+// The empty asm re-defines the offset at each access: a loop-invariant
+// offset would otherwise reach the loop already widened to 64 bits, and the
+// compiler then forms a 64-bit lane address per access instead of using the
+// scalar-base form of the instruction.
 template <class S>
 __device__ __forceinline__ S ld_at(const S* base, uint32_t off) {
+  asm volatile("" : "+v"(off));
   return *(const S*)((const char*)base + off);
 }
 template <class S>
 __device__ __forceinline__ void st_at(S* base, uint32_t off, S v) {
+  asm volatile("" : "+v"(off));
   *(S*)((char*)base + off) = v;
 }
 
@@ -112,10 +153,19 @@ __device__ __forceinline__ LdsCell<S, W> lds_cell(const S* ring, int plane_size,
   return r;
 }
 
-__device__ __forceinline__ int s2(int p) { return p & 1; }
-__device__ __forceinline__ int s3(int p) { return (p + 6) % 3; }
-__device__ __forceinline__ int s4(int p) { return p & 3; }
+// ring slot of plane p in a ring of D planes (p >= -6)
+template <int D>
+__device__ __forceinline__ int slot(int p) {
+  if constexpr (D == 2) return p & 1;
+  else if constexpr (D == 3) return (p + 6) % 3;
+  else return p & 3;
+}
 
+// Ghost-cell handling of a stage launch, fixed at compile time where the
+// launch knows it: WRAP_STORED (ghosts are stored), WRAP_PERIODIC (every
+// axis periodic, no ghost is read or written), WRAP_RUNTIME (A.wrap
+// decides; the batched kernel mixes both kinds of instances).
+constexpr int WRAP_STORED = 0, WRAP_PERIODIC = 1, WRAP_RUNTIME = 2;
 
 // Planes k0..k1-1 of the TX x TY column at (i0, j0) of a stage.
 //
@@ -127,30 +177,34 @@ __device__ __forceinline__ int s4(int p) { return p & 3; }
 // Skipped cells are not written, so the in-place stage 3 leaves their U^n
 // for the fix-up to read.  Without the redo code the kernel fits four waves
 // per SIMD.
-template <class PR, class TL, bool FIX>
+template <class PR, class TL, bool FIX, int WM>
 __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typename PR::P>& A, int i0, int j0,
                                           int k0, int k1, int tile_id, unsigned char* smem_raw) {
   using S = typename PR::S;
   using T = typename PR::T;
   using Q = typename PR::Q;
   using P = typename PR::P;
+  using RG = Ring<PR, FIX>;
   constexpr int TX = TL::TX, TY = TL::TY, W2 = TL::W2, N2 = TL::N2, W1 = TL::W1, N1 = TL::N1;
   constexpr int NT = TL::NT, L2PT = TL::L2PT;
-  S* rex = (S*)smem_raw;         // [2][N2]
-  S* rey = rex + 2 * N2;         // [2][N2]
-  S* rez = rey + 2 * N2;         // [3][N2]
-  S* rbx = rez + 3 * N2;         // [4][N2]
-  S* rby = rbx + 4 * N2;         // [4][N2]
-  S* rbz = rby + 4 * N2;         // [3][N2]
-  S* cex = rbz + 3 * N2;         // [2][N1]
-  S* cey = cex + 2 * N1;         // [2][N1]
-  S* cez = cey + 2 * N1;         // [3][N1]
-  unsigned* nflag = (unsigned*)(cez + 3 * N1);  // [4]: plane q in slot q & 3
+  constexpr bool TWO = RG::TWO;
+  S* rex = (S*)smem_raw;
+  S* rey = rex + RG::EX * N2;
+  S* rez = rey + RG::EY * N2;
+  S* rbx = rez + RG::EZ * N2;
+  S* rby = rbx + RG::BX * N2;
+  S* rbz = rby + RG::BY * N2;
+  S* cex = rbz + RG::BZ * N2;  // [2][N1]
+  S* cey = cex + 2 * N1;       // [2][N1]
+  S* cez = cey + 2 * N1;       // [3][N1]
+  unsigned* nflag = (unsigned*)(cez + 3 * N1);  // plane q in slot q % FLAGS
+  constexpr unsigned FM = RG::FLAGS - 1;
 
   const Geom& g = A.g;
   const int64_t cs = g.cs;
   const int tid = threadIdx.x;
-  if (FIX && tid < 4) nflag[tid] = 0u;
+  const bool wrap = WM == WRAP_RUNTIME ? A.wrap != 0 : WM == WRAP_PERIODIC;
+  if (FIX && tid < RG::FLAGS) nflag[tid] = 0u;
 
   // x-y byte offsets (from the plane origin) of this thread's halo cells
   // of the raw planes: wrapped on periodic boxes, clamped to the stored
@@ -160,7 +214,7 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
   for (int m = 0; m < L2PT; ++m) {
     const int c = min(tid + m * NT, N2 - 1);
     int gi = i0 - 2 + c % W2, gj = j0 - 2 + c / W2;
-    if (A.wrap) {
+    if (wrap) {
       gi = wrap_index(gi, g.nx);
       gj = wrap_index(gj, g.ny);
     } else {
@@ -171,7 +225,7 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
   }
   // uniform pointer to plane p of a buffer's component 0
   auto plane_of = [&](const S* base, int p) -> const S* {
-    const int gk = A.wrap ? wrap_index(p, g.nz) : p;
+    const int gk = wrap ? wrap_index(p, g.nz) : p;
     return base + (int64_t)gk * g.plane;
   };
 
@@ -201,73 +255,89 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
     for (int m = 0; m < L2PT; ++m) {
       const int c = tid + m * NT;
       if (c < N2) {
-        rex[s2(p) * N2 + c] = reg[m][0];
-        rey[s2(p) * N2 + c] = reg[m][1];
-        rez[s3(p) * N2 + c] = reg[m][2];
-        rbx[s4(p) * N2 + c] = reg[m][3];
-        rby[s4(p) * N2 + c] = reg[m][4];
-        rbz[s3(p) * N2 + c] = reg[m][5];
+        rex[slot<RG::EX>(p) * N2 + c] = reg[m][0];
+        rey[slot<RG::EY>(p) * N2 + c] = reg[m][1];
+        rez[slot<RG::EZ>(p) * N2 + c] = reg[m][2];
+        rbx[slot<RG::BX>(p) * N2 + c] = reg[m][3];
+        rby[slot<RG::BY>(p) * N2 + c] = reg[m][4];
+        rbz[slot<RG::BZ>(p) * N2 + c] = reg[m][5];
       }
     }
+  };
+  // raw field accessors around cell lc of the raw planes, at plane q
+  auto raw_cells = [&](int q, int lc, auto&& f) {
+    const auto lex = lds_cell<S, W2>(rex, N2, q, lc, slot<RG::EX>), ley = lds_cell<S, W2>(rey, N2, q, lc, slot<RG::EY>);
+    const auto lez = lds_cell<S, W2>(rez, N2, q, lc, slot<RG::EZ>), lbx = lds_cell<S, W2>(rbx, N2, q, lc, slot<RG::BX>);
+    const auto lby = lds_cell<S, W2>(rby, N2, q, lc, slot<RG::BY>), lbz = lds_cell<S, W2>(rbz, N2, q, lc, slot<RG::BZ>);
+    f(lex, ley, lez, lbx, lby, lbz);
   };
 
   // The B interpolants and raw |B|^2 of a thread's own cell serve both the
   // clean of that cell and, one plane later, its drift currents (they are
   // the same expressions of the same stage-input B).  FP32 policies keep
   // them in registers between the two; FP64 recomputes them (register
-  // budget).
-  constexpr bool REUSE = sizeof(Q) == 4;
+  // budget).  With TWO the own clean also keeps the raw-B curl brackets and
+  // the cell's own raw B for the push.
+  constexpr bool REUSE = RG::REUSE;
   constexpr int RING = 2 * W1 + 2 * TY;  // halo cells of the cleaned region
   BBar<Q> nbb{}, cbb{};
   Q nbref = (Q)0, cbref = (Q)0;
+  T ncb[3] = {(T)0, (T)0, (T)0}, ccb[3] = {(T)0, (T)0, (T)0};
+  S nbo[3] = {(S)0, (S)0, (S)0}, cbo[3] = {(S)0, (S)0, (S)0};
 
   // cleanEpar of cell (mx, my) of the cleaned region of plane q
   auto clean_at = [&](int q, int mx, int my, bool own) {
     const int c = my * W1 + mx;
     const int lc = (my + 1) * W2 + (mx + 1);  // same cell in the raw plane
     const int gi = i0 - 1 + mx, gj = j0 - 1 + my;
-    const auto lex = lds_cell<S, W2>(rex, N2, q, lc, s2), ley = lds_cell<S, W2>(rey, N2, q, lc, s2);
-    const auto lez = lds_cell<S, W2>(rez, N2, q, lc, s3), lbx = lds_cell<S, W2>(rbx, N2, q, lc, s4);
-    const auto lby = lds_cell<S, W2>(rby, N2, q, lc, s4), lbz = lds_cell<S, W2>(rbz, N2, q, lc, s3);
-    auto EX = [&](int a, int b, int d) { return (T)lex(a, b, d); };
-    auto EY = [&](int a, int b, int d) { return (T)ley(a, b, d); };
-    auto EZ = [&](int a, int b, int d) { return (T)lez(a, b, d); };
-    auto BXF = [&](int a, int b, int d) { return (T)lbx(a, b, d); };
-    auto BYF = [&](int a, int b, int d) { return (T)lby(a, b, d); };
-    auto BZF = [&](int a, int b, int d) { return (T)lbz(a, b, d); };
-    S ox, oy, oz;
-    const bool ghost = !A.wrap && (gi < 0 || gi >= g.nx || gj < 0 || gj >= g.ny || q < 0 || q >= g.nz);
-    const bool project = A.clean_in && !ghost;
-    BBar<Q> bb{};
-    Q bref = (Q)0;
-    if (project || (REUSE && own)) {
-      bb = bbar<T, Q, PR::null_fp64>(BXF, BYF, BZF);
-      if (PR::null_fp64) bref = cell_b2<Q>(BXF, BYF, BZF);
-      if (FIX && near_null_cell<Q>(bb, bref)) nflag[(unsigned)q & 3u] = 1u;
-    }
-    if (!A.clean_in) {
-      ox = lex(0, 0, 0);
-      oy = ley(0, 0, 0);
-      oz = lez(0, 0, 0);
-    } else if (ghost) {
-      const int64_t a = g.at(min(gi, g.nx + NG - 1), min(gj, g.ny + NG - 1), q);
-      ox = A.ge[a];
-      oy = A.ge[a + cs];
-      oz = A.ge[a + 2 * cs];
-    } else {
-      P px, py, pz;
-      clean_cell_bb<PR, PR::null_fp64 && !FIX>(EX, EY, EZ, BXF, BYF, BZF, bb, bref, A.guard, px, py, pz);
-      ox = (S)px;
-      oy = (S)py;
-      oz = (S)pz;
-    }
-    cex[s2(q) * N1 + c] = ox;
-    cey[s2(q) * N1 + c] = oy;
-    cez[s3(q) * N1 + c] = oz;
-    if (REUSE && own) {
-      nbb = bb;
-      nbref = bref;
-    }
+    raw_cells(q, lc, [&](const auto& lex, const auto& ley, const auto& lez, const auto& lbx, const auto& lby,
+                         const auto& lbz) {
+      auto EX = [&](int a, int b, int d) { return (T)lex(a, b, d); };
+      auto EY = [&](int a, int b, int d) { return (T)ley(a, b, d); };
+      auto EZ = [&](int a, int b, int d) { return (T)lez(a, b, d); };
+      auto BXF = [&](int a, int b, int d) { return (T)lbx(a, b, d); };
+      auto BYF = [&](int a, int b, int d) { return (T)lby(a, b, d); };
+      auto BZF = [&](int a, int b, int d) { return (T)lbz(a, b, d); };
+      S ox, oy, oz;
+      const bool ghost = !wrap && (gi < 0 || gi >= g.nx || gj < 0 || gj >= g.ny || q < 0 || q >= g.nz);
+      const bool project = A.clean_in && !ghost;
+      BBar<Q> bb{};
+      Q bref = (Q)0;
+      if (project || (REUSE && own)) {
+        bb = bbar<T, Q, PR::null_fp64>(BXF, BYF, BZF);
+        if (PR::null_fp64) bref = cell_b2<Q>(BXF, BYF, BZF);
+        if (FIX && near_null_cell<Q>(bb, bref)) nflag[(unsigned)q & FM] = 1u;
+      }
+      if (!A.clean_in) {
+        ox = lex(0, 0, 0);
+        oy = ley(0, 0, 0);
+        oz = lez(0, 0, 0);
+      } else if (ghost) {
+        const int64_t a = g.at(min(gi, g.nx + NG - 1), min(gj, g.ny + NG - 1), q);
+        ox = A.ge[a];
+        oy = A.ge[a + cs];
+        oz = A.ge[a + 2 * cs];
+      } else {
+        P px, py, pz;
+        clean_cell_bb<PR, PR::null_fp64 && !FIX>(EX, EY, EZ, BXF, BYF, BZF, bb, bref, A.guard, px, py, pz);
+        ox = (S)px;
+        oy = (S)py;
+        oz = (S)pz;
+      }
+      cex[slot<2>(q) * N1 + c] = ox;
+      cey[slot<2>(q) * N1 + c] = oy;
+      cez[slot<3>(q) * N1 + c] = oz;
+      if (REUSE && own) {
+        nbb = bb;
+        nbref = bref;
+        if constexpr (TWO) {
+          curl_b<T>(BXF, BYF, BZF, ncb);
+          nbo[0] = lbx(0, 0, 0);
+          nbo[1] = lby(0, 0, 0);
+          nbo[2] = lbz(0, 0, 0);
+        }
+      }
+    });
   };
   // cleanEpar of plane q on the tile and its 1-cell halo: each thread its
   // own cell, then the first RING threads the halo ring
@@ -291,6 +361,17 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
       clean_at(q, mx, my, false);
     }
   };
+  auto hand_over = [&]() {
+    cbb = nbb;
+    cbref = nbref;
+    if constexpr (TWO) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        ccb[m] = ncb[m];
+        cbo[m] = nbo[m];
+      }
+    }
+  };
 
   // prologue: planes k0-2 .. k0+1 raw, k0-1 and k0 cleaned
   load_plane(k0 - 2);
@@ -307,8 +388,7 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
   __syncthreads();
   clean_plane(k0);
   __syncthreads();
-  cbb = nbb;
-  cbref = nbref;
+  hand_over();
   if (k0 < k1) load_plane(k0 + 2);
 
   const int tx = tid % TX, ty = tid / TX;
@@ -322,10 +402,12 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
 
   unsigned long long skipped = 0ull;  // FIX: bit k - k0 for a skipped plane k
   for (int k = k0; k < k1; ++k) {
-    // raw plane k+2 arrived during the previous iteration
+    // raw plane k+2 arrived during the previous iteration (with TWO, other
+    // waves may still push plane k-1: it reads no raw slot)
     store_plane(k + 2);
-    // the flag slot of plane k+2 (last read for plane k-2's output)
-    if (FIX && tid == 0) nflag[(unsigned)(k + 2) & 3u] = 0u;
+    // the flag slot of plane k+2 (its previous plane was last read before
+    // this iteration)
+    if (FIX && tid == 0) nflag[(unsigned)(k + 2) & FM] = 0u;
     const int64_t pk = (int64_t)k * g.plane;
     // U^n and the B0 curl of plane k, then raw plane k+3, in flight while
     // this iteration cleans and computes (FP64 loads U^n where it is used:
@@ -358,33 +440,46 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
     __syncthreads();
     bool skip = false;
     if constexpr (FIX) {
-      const unsigned f = nflag[(unsigned)(k - 1) & 3u] | nflag[(unsigned)k & 3u] | nflag[(unsigned)(k + 1) & 3u];
+      const unsigned f = nflag[(unsigned)(k - 1) & FM] | nflag[(unsigned)k & FM] | nflag[(unsigned)(k + 1) & FM];
       skip = __builtin_amdgcn_readfirstlane(f) != 0u;
       if (skip) skipped |= 1ull << (k - k0);
     }
     if (active && !skip) {
       if (!EARLY) load_un();
-      const auto lex = lds_cell<S, W1>(cex, N1, k, lc1, s2), ley = lds_cell<S, W1>(cey, N1, k, lc1, s2);
-      const auto lez = lds_cell<S, W1>(cez, N1, k, lc1, s3);
-      const auto lbx = lds_cell<S, W2>(rbx, N2, k, lc2, s4), lby = lds_cell<S, W2>(rby, N2, k, lc2, s4);
-      const auto lbz = lds_cell<S, W2>(rbz, N2, k, lc2, s3);
+      const auto lex = lds_cell<S, W1>(cex, N1, k, lc1, slot<2>), ley = lds_cell<S, W1>(cey, N1, k, lc1, slot<2>);
+      const auto lez = lds_cell<S, W1>(cez, N1, k, lc1, slot<3>);
       auto EX = [&](int a, int b, int d) { return (T)lex(a, b, d); };
       auto EY = [&](int a, int b, int d) { return (T)ley(a, b, d); };
       auto EZ = [&](int a, int b, int d) { return (T)lez(a, b, d); };
-      auto BXF = [&](int a, int b, int d) { return (T)lbx(a, b, d); };
-      auto BYF = [&](int a, int b, int d) { return (T)lby(a, b, d); };
-      auto BZF = [&](int a, int b, int d) { return (T)lbz(a, b, d); };
       // computeRho at the four nodes the currents read, from the cleaned
       // rings (cheaper than a separate rho pass and its barrier)
       auto RHO = [&](int a, int b, int d) { return rho_node<T>(EX, EY, EZ, a, b, d); };
       P d[6];
-      if constexpr (REUSE)
-        rhs_cell_bb<PR, PR::null_fp64 && !FIX>(EX, EY, EZ, BXF, BYF, BZF, RHO, cbb, cbref, A.cc, tt[0], tt[1],
-                                               tt[2], d);
-      else
-        rhs_cell<PR>(EX, EY, EZ, BXF, BYF, BZF, RHO, A.cc, tt[0], tt[1], tt[2], d);
-      P v[6] = {(P)lex(0, 0, 0), (P)ley(0, 0, 0), (P)lez(0, 0, 0),
-                (P)lbx(0, 0, 0), (P)lby(0, 0, 0), (P)lbz(0, 0, 0)};
+      P v[6];
+      v[0] = (P)lex(0, 0, 0);
+      v[1] = (P)ley(0, 0, 0);
+      v[2] = (P)lez(0, 0, 0);
+      if constexpr (TWO) {
+        rhs_cell_pre<PR>(EX, EY, EZ, RHO, cbb, cbref, A.cc, tt[0], tt[1], tt[2], ccb, d);
+        v[3] = (P)cbo[0];
+        v[4] = (P)cbo[1];
+        v[5] = (P)cbo[2];
+      } else {
+        const auto lbx = lds_cell<S, W2>(rbx, N2, k, lc2, slot<RG::BX>);
+        const auto lby = lds_cell<S, W2>(rby, N2, k, lc2, slot<RG::BY>);
+        const auto lbz = lds_cell<S, W2>(rbz, N2, k, lc2, slot<RG::BZ>);
+        auto BXF = [&](int a, int b, int d) { return (T)lbx(a, b, d); };
+        auto BYF = [&](int a, int b, int d) { return (T)lby(a, b, d); };
+        auto BZF = [&](int a, int b, int d) { return (T)lbz(a, b, d); };
+        if constexpr (REUSE)
+          rhs_cell_bb<PR, PR::null_fp64 && !FIX>(EX, EY, EZ, BXF, BYF, BZF, RHO, cbb, cbref, A.cc, tt[0], tt[1],
+                                                 tt[2], d);
+        else
+          rhs_cell<PR>(EX, EY, EZ, BXF, BYF, BZF, RHO, A.cc, tt[0], tt[1], tt[2], d);
+        v[3] = (P)lbx(0, 0, 0);
+        v[4] = (P)lby(0, 0, 0);
+        v[5] = (P)lbz(0, 0, 0);
+      }
       P u[6];
       if (A.stage1) {
         for (int m = 0; m < 6; ++m) u[m] = v[m];
@@ -407,25 +502,24 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
       st_at(ob + cs, oxy, (S)v[4]);
       st_at(ob + 2 * cs, oxy, (S)v[5]);
     }
-    cbb = nbb;
-    cbref = nbref;
-    __syncthreads();  // plane k's rings are free for plane k+3 / k+2
+    hand_over();
+    if constexpr (!TWO) __syncthreads();  // plane k's rings are free for plane k+3 / k+2
   }
   if (FIX && tid == 0) A.fixmask[tile_id] = skipped;
 }
 
 // One tile (tile_id in x-fastest, then y, then z-chunk order) of a stage.
-template <class PR, class TL, bool FIX = false>
+template <class PR, class TL, bool FIX, int WM>
 __device__ __forceinline__ void stage_tile(const StageArgs<typename PR::S, typename PR::P>& A, int tile_id,
                                            unsigned char* smem_raw) {
   const int bxi = tile_id % A.nbx;
   const int byi = (tile_id / A.nbx) % A.nby;
   const int bzi = tile_id / (A.nbx * A.nby);
   const int k0 = bzi * A.kchunk;
-  stage_run<PR, TL, FIX>(A, bxi * TL::TX, byi * TL::TY, k0, min(k0 + A.kchunk, A.g.nz), tile_id, smem_raw);
+  stage_run<PR, TL, FIX, WM>(A, bxi * TL::TX, byi * TL::TY, k0, min(k0 + A.kchunk, A.g.nz), tile_id, smem_raw);
 }
 
-template <class PR, class TL>
+template <class PR, class TL, int WM>
 __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(sizeof(typename PR::S) == 4 ? 4 : 2)))
     k_stage(StageArgs<typename PR::S, typename PR::P> A) {
   extern __shared__ unsigned char smem_raw[];
@@ -435,14 +529,14 @@ __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(siz
   const int per = (A.nblocks + 7) / 8;
   const int tile_id = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
   if (tile_id >= A.nblocks) return;
-  stage_tile<PR, TL, PR::null_fp64>(A, tile_id, smem_raw);
+  stage_tile<PR, TL, PR::null_fp64, WM>(A, tile_id, smem_raw);
 }
 
 // The planes of tile t that a FIX stage launch marked in `m`, with the
 // near-null FP64 redo: the same per-cell code as the stage kernel of the
 // unfused path, so the results are those of a kernel that redoes inline.
 // Each run of consecutive marked planes is one stage_run.
-template <class PR, class TL>
+template <class PR, class TL, int WM>
 __device__ __forceinline__ void fix_tile(const StageArgs<typename PR::S, typename PR::P>& A, int t,
                                          unsigned long long m, unsigned char* smem_raw) {
   const int i0 = (t % A.nbx) * TL::TX;
@@ -452,19 +546,19 @@ __device__ __forceinline__ void fix_tile(const StageArgs<typename PR::S, typenam
     const int a = __builtin_ctzll(m);  // first marked plane
     const unsigned long long rest = ~(m >> a);
     const int len = rest == 0ull ? 64 - a : __builtin_ctzll(rest);  // marked planes from a on
-    stage_run<PR, TL, false>(A, i0, j0, k0 + a, k0 + a + len, t, smem_raw);
+    stage_run<PR, TL, false, WM>(A, i0, j0, k0 + a, k0 + a + len, t, smem_raw);
     m = (a + len >= 64) ? 0ull : (m >> (a + len)) << (a + len);
   }
 }
 
 // After a FIX k_stage launch: block b takes tiles b, b + gridDim.x, ...
-template <class PR, class TL>
+template <class PR, class TL, int WM>
 __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(2)))
     k_stage_fix(StageArgs<typename PR::S, typename PR::P> A) {
   extern __shared__ unsigned char smem_raw[];
   for (int t = (int)blockIdx.x; t < A.nblocks; t += (int)gridDim.x) {
     const unsigned long long m = A.fixmask[t];
-    if (m != 0ull) fix_tile<PR, TL>(A, t, m, smem_raw);
+    if (m != 0ull) fix_tile<PR, TL, WM>(A, t, m, smem_raw);
   }
 }
 
