@@ -189,7 +189,8 @@ int tff_stage_kernel_time(tff_handle* h, double* total_ms, long* launches);
  * give the same numbers bit for bit. */
 int tff_set_path(tff_handle* h, int fused);
 /* Tile shape (index into the built shapes; -1 = let the autotuner pick on
- * the next step) and z-chunk (1..64) of the fused stage kernel, and the
+ * the next step) and z-chunk (a multiple of 4 in 4..64) of the fused stage
+ * kernel, and the
  * current choice (tile index, its x and y extent, z-chunk). */
 int tff_set_tuning(tff_handle* h, int tile, int kchunk);
 int tff_get_tuning(tff_handle* h, int* tile, int* tx, int* ty, int* kchunk);

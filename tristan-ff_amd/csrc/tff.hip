@@ -1203,8 +1203,8 @@ int tff_set_path(tff_handle* h, int fused) {
 
 int tff_set_tuning(tff_handle* h, int tile, int kchunk) {
   if (!h) return TFF_EINVAL;
-  if (tile < -1 || tile >= NTILES || (tile >= 0 && (kchunk < 1 || kchunk > 64)))
-    return fail(h, TFF_EINVAL, "bad tuning (tile index out of range, or z-chunk not in 1..64)");
+  if (tile < -1 || tile >= NTILES || (tile >= 0 && (kchunk < 4 || kchunk > 64 || kchunk % 4 != 0)))
+    return fail(h, TFF_EINVAL, "bad tuning (tile index out of range, or z-chunk not a multiple of 4 in 4..64)");
   if (tile >= 0) {
     size_t lds = 0;
     dispatch<TileLds>(h, tile, &lds);

@@ -38,6 +38,8 @@
 // raw reads take the stored ghosts, and the cleaned value of a ghost cell
 // comes from `ge`, which the boundary passes fill after each stage.
 #pragma once
+#include <type_traits>
+
 #include "tff_kernels.h"
 
 namespace tff {
@@ -55,20 +57,29 @@ struct Tile {
 //
 // The clean of plane q reads raw ex, ey at q..q+1, ez at q-1..q, bx, by at
 // q-1..q and bz at q..q+1; the push of plane k reads the cleaned E at
-// k-1..k+1.  TWO (FP32 policies without the inline FP64 redo): the push
-// also takes its raw-B terms (the curl brackets and the cell's own B) from
-// registers that the own-cell clean filled one plane earlier, so it reads
-// no raw plane at all.  Raw plane k+3 may then be stored while other waves
-// still push plane k, and a plane needs two barriers.  Otherwise the push
-// reads raw B at k-1..k+1, and a third barrier protects the rings.
+// k-1..k+1.  TWO: the push also takes its raw-B terms (the B interpolants,
+// the curl brackets and the cell's own B) from registers that the own-cell
+// clean filled one plane earlier, so it reads no raw plane at all.  Raw
+// plane k+3 may then be stored while other waves still push plane k, and a
+// plane needs two barriers.  Otherwise (the FP32-bulk fix-up, which redoes
+// near-null cells from the raw B) the push reads raw B at k-1..k+1 and a
+// third barrier protects the rings.
+//
+// STATIC (FP32 storage with TWO): every ring holds 2 or 4 planes, so with
+// the z loop unrolled by four (z-chunks start at multiples of 4) every ring
+// slot is a compile-time constant: LDS addresses are one base register
+// plus immediate offsets, and the own-cell registers alternate between two
+// sets instead of being copied.  FP64 keeps rings of 3 (the LDS of a
+// second workgroup per CU) and runtime slots.
 template <class PR, bool FIX>
 struct Ring {
-  static constexpr bool REUSE = sizeof(typename PR::Q) == 4;
-  static constexpr bool TWO = REUSE && !(PR::null_fp64 && !FIX);
-  static constexpr int EX = 2, EY = 2, EZ = 3;
-  static constexpr int BX = TWO ? 3 : 4, BY = TWO ? 3 : 4, BZ = TWO ? 2 : 3;
+  static constexpr bool TWO = !(PR::null_fp64 && !FIX);
+  static constexpr bool STATIC = TWO && sizeof(typename PR::S) == 4;
+  static constexpr int EX = 2, EY = 2, EZ = STATIC ? 4 : 3;
+  static constexpr int BX = STATIC ? 4 : (TWO ? 3 : 4), BY = BX, BZ = TWO ? 2 : 3;
+  static constexpr int CX = 2, CY = 2, CZ = STATIC ? 4 : 3;
   static constexpr int RAW = EX + EY + EZ + BX + BY + BZ;
-  static constexpr int CLEAN = 2 + 2 + 3;
+  static constexpr int CLEAN = CX + CY + CZ;
   // near-null flags of the planes (FP32-bulk), one word per plane slot
   static constexpr int FLAGS = TWO ? 8 : 4;
   template <class TL>
@@ -194,10 +205,10 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
   S* rbx = rez + RG::EZ * N2;
   S* rby = rbx + RG::BX * N2;
   S* rbz = rby + RG::BY * N2;
-  S* cex = rbz + RG::BZ * N2;  // [2][N1]
-  S* cey = cex + 2 * N1;       // [2][N1]
-  S* cez = cey + 2 * N1;       // [3][N1]
-  unsigned* nflag = (unsigned*)(cez + 3 * N1);  // plane q in slot q % FLAGS
+  S* cex = rbz + RG::BZ * N2;
+  S* cey = cex + RG::CX * N1;
+  S* cez = cey + RG::CY * N1;
+  unsigned* nflag = (unsigned*)(cez + RG::CZ * N1);  // plane q in slot q % FLAGS
   constexpr unsigned FM = RG::FLAGS - 1;
 
   const Geom& g = A.g;
@@ -229,6 +240,9 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
     return base + (int64_t)gk * g.plane;
   };
 
+  // Planes are addressed twice: `q` (absolute: global rows, ghost tests,
+  // flags) and `qs` (ring slots).  With STATIC rings qs is q minus a
+  // multiple of 4 known at compile time, which leaves every slot the same.
   S reg[L2PT][6];
   auto load_plane = [&](int p) {
     const S* e0 = plane_of(A.ye, p);
@@ -250,99 +264,106 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
       }
     }
   };
-  auto store_plane = [&](int p) {
+  auto store_plane = [&](int ps) {
 #pragma unroll
     for (int m = 0; m < L2PT; ++m) {
       const int c = tid + m * NT;
       if (c < N2) {
-        rex[slot<RG::EX>(p) * N2 + c] = reg[m][0];
-        rey[slot<RG::EY>(p) * N2 + c] = reg[m][1];
-        rez[slot<RG::EZ>(p) * N2 + c] = reg[m][2];
-        rbx[slot<RG::BX>(p) * N2 + c] = reg[m][3];
-        rby[slot<RG::BY>(p) * N2 + c] = reg[m][4];
-        rbz[slot<RG::BZ>(p) * N2 + c] = reg[m][5];
+        rex[slot<RG::EX>(ps) * N2 + c] = reg[m][0];
+        rey[slot<RG::EY>(ps) * N2 + c] = reg[m][1];
+        rez[slot<RG::EZ>(ps) * N2 + c] = reg[m][2];
+        rbx[slot<RG::BX>(ps) * N2 + c] = reg[m][3];
+        rby[slot<RG::BY>(ps) * N2 + c] = reg[m][4];
+        rbz[slot<RG::BZ>(ps) * N2 + c] = reg[m][5];
       }
     }
-  };
-  // raw field accessors around cell lc of the raw planes, at plane q
-  auto raw_cells = [&](int q, int lc, auto&& f) {
-    const auto lex = lds_cell<S, W2>(rex, N2, q, lc, slot<RG::EX>), ley = lds_cell<S, W2>(rey, N2, q, lc, slot<RG::EY>);
-    const auto lez = lds_cell<S, W2>(rez, N2, q, lc, slot<RG::EZ>), lbx = lds_cell<S, W2>(rbx, N2, q, lc, slot<RG::BX>);
-    const auto lby = lds_cell<S, W2>(rby, N2, q, lc, slot<RG::BY>), lbz = lds_cell<S, W2>(rbz, N2, q, lc, slot<RG::BZ>);
-    f(lex, ley, lez, lbx, lby, lbz);
   };
 
   // The B interpolants and raw |B|^2 of a thread's own cell serve both the
   // clean of that cell and, one plane later, its drift currents (they are
-  // the same expressions of the same stage-input B).  FP32 policies keep
-  // them in registers between the two; FP64 recomputes them (register
-  // budget).  With TWO the own clean also keeps the raw-B curl brackets and
-  // the cell's own raw B for the push.
-  constexpr bool REUSE = RG::REUSE;
-  constexpr int RING = 2 * W1 + 2 * TY;  // halo cells of the cleaned region
-  BBar<Q> nbb{}, cbb{};
-  Q nbref = (Q)0, cbref = (Q)0;
-  T ncb[3] = {(T)0, (T)0, (T)0}, ccb[3] = {(T)0, (T)0, (T)0};
-  S nbo[3] = {(S)0, (S)0, (S)0}, cbo[3] = {(S)0, (S)0, (S)0};
+  // the same expressions of the same stage-input B); with TWO the own clean
+  // also keeps the raw-B curl brackets and the cell's own raw B for the
+  // push.  STATIC: set (q & 1) holds cleaned plane q's; otherwise the
+  // clean fills set 1 and the push reads set 0.
+  struct Own {
+    BBar<Q> bb;
+    Q bref;
+    T cb[3];
+    S bo[3];
+  };
+  Own own0{}, own1{};
+  // register set i (a compile-time choice, so both stay in registers)
+  auto own_set = [&](auto i) -> Own& {
+    if constexpr (decltype(i)::value == 0) return own0;
+    else return own1;
+  };
 
-  // cleanEpar of cell (mx, my) of the cleaned region of plane q
-  auto clean_at = [&](int q, int mx, int my, bool own) {
+  constexpr int RING = 2 * W1 + 2 * TY;  // halo cells of the cleaned region
+
+  // cleanEpar of cell (mx, my) of the cleaned region of plane q; an own
+  // cell (OWN) also fills `dst`
+  auto clean_at = [&](int q, int qs, int mx, int my, auto own_tag, Own& dst) {
+    constexpr bool OWN = decltype(own_tag)::value;
     const int c = my * W1 + mx;
     const int lc = (my + 1) * W2 + (mx + 1);  // same cell in the raw plane
     const int gi = i0 - 1 + mx, gj = j0 - 1 + my;
-    raw_cells(q, lc, [&](const auto& lex, const auto& ley, const auto& lez, const auto& lbx, const auto& lby,
-                         const auto& lbz) {
-      auto EX = [&](int a, int b, int d) { return (T)lex(a, b, d); };
-      auto EY = [&](int a, int b, int d) { return (T)ley(a, b, d); };
-      auto EZ = [&](int a, int b, int d) { return (T)lez(a, b, d); };
-      auto BXF = [&](int a, int b, int d) { return (T)lbx(a, b, d); };
-      auto BYF = [&](int a, int b, int d) { return (T)lby(a, b, d); };
-      auto BZF = [&](int a, int b, int d) { return (T)lbz(a, b, d); };
-      S ox, oy, oz;
-      const bool ghost = !wrap && (gi < 0 || gi >= g.nx || gj < 0 || gj >= g.ny || q < 0 || q >= g.nz);
-      const bool project = A.clean_in && !ghost;
-      BBar<Q> bb{};
-      Q bref = (Q)0;
-      if (project || (REUSE && own)) {
-        bb = bbar<T, Q, PR::null_fp64>(BXF, BYF, BZF);
-        if (PR::null_fp64) bref = cell_b2<Q>(BXF, BYF, BZF);
-        if (FIX && near_null_cell<Q>(bb, bref)) nflag[(unsigned)q & FM] = 1u;
+    const auto lex = lds_cell<S, W2>(rex, N2, qs, lc, slot<RG::EX>);
+    const auto ley = lds_cell<S, W2>(rey, N2, qs, lc, slot<RG::EY>);
+    const auto lez = lds_cell<S, W2>(rez, N2, qs, lc, slot<RG::EZ>);
+    const auto lbx = lds_cell<S, W2>(rbx, N2, qs, lc, slot<RG::BX>);
+    const auto lby = lds_cell<S, W2>(rby, N2, qs, lc, slot<RG::BY>);
+    const auto lbz = lds_cell<S, W2>(rbz, N2, qs, lc, slot<RG::BZ>);
+    auto EX = [&](int a, int b, int d) { return (T)lex(a, b, d); };
+    auto EY = [&](int a, int b, int d) { return (T)ley(a, b, d); };
+    auto EZ = [&](int a, int b, int d) { return (T)lez(a, b, d); };
+    auto BXF = [&](int a, int b, int d) { return (T)lbx(a, b, d); };
+    auto BYF = [&](int a, int b, int d) { return (T)lby(a, b, d); };
+    auto BZF = [&](int a, int b, int d) { return (T)lbz(a, b, d); };
+    S ox, oy, oz;
+    const bool ghost = !wrap && (gi < 0 || gi >= g.nx || gj < 0 || gj >= g.ny || q < 0 || q >= g.nz);
+    const bool project = A.clean_in && !ghost;
+    BBar<Q> bb{};
+    Q bref = (Q)0;
+    if (project || OWN) {
+      bb = bbar<T, Q, PR::null_fp64>(BXF, BYF, BZF);
+      if (PR::null_fp64) bref = cell_b2<Q>(BXF, BYF, BZF);
+      if (FIX && near_null_cell<Q>(bb, bref)) nflag[(unsigned)q & FM] = 1u;
+    }
+    if (!A.clean_in) {
+      ox = lex(0, 0, 0);
+      oy = ley(0, 0, 0);
+      oz = lez(0, 0, 0);
+    } else if (ghost) {
+      const int64_t a = g.at(min(gi, g.nx + NG - 1), min(gj, g.ny + NG - 1), q);
+      ox = A.ge[a];
+      oy = A.ge[a + cs];
+      oz = A.ge[a + 2 * cs];
+    } else {
+      P px, py, pz;
+      clean_cell_bb<PR, PR::null_fp64 && !FIX>(EX, EY, EZ, BXF, BYF, BZF, bb, bref, A.guard, px, py, pz);
+      ox = (S)px;
+      oy = (S)py;
+      oz = (S)pz;
+    }
+    cex[slot<RG::CX>(qs) * N1 + c] = ox;
+    cey[slot<RG::CY>(qs) * N1 + c] = oy;
+    cez[slot<RG::CZ>(qs) * N1 + c] = oz;
+    if constexpr (OWN) {
+      dst.bb = bb;
+      dst.bref = bref;
+      if constexpr (TWO) {
+        curl_b<T>(BXF, BYF, BZF, dst.cb);
+        dst.bo[0] = lbx(0, 0, 0);
+        dst.bo[1] = lby(0, 0, 0);
+        dst.bo[2] = lbz(0, 0, 0);
       }
-      if (!A.clean_in) {
-        ox = lex(0, 0, 0);
-        oy = ley(0, 0, 0);
-        oz = lez(0, 0, 0);
-      } else if (ghost) {
-        const int64_t a = g.at(min(gi, g.nx + NG - 1), min(gj, g.ny + NG - 1), q);
-        ox = A.ge[a];
-        oy = A.ge[a + cs];
-        oz = A.ge[a + 2 * cs];
-      } else {
-        P px, py, pz;
-        clean_cell_bb<PR, PR::null_fp64 && !FIX>(EX, EY, EZ, BXF, BYF, BZF, bb, bref, A.guard, px, py, pz);
-        ox = (S)px;
-        oy = (S)py;
-        oz = (S)pz;
-      }
-      cex[slot<2>(q) * N1 + c] = ox;
-      cey[slot<2>(q) * N1 + c] = oy;
-      cez[slot<3>(q) * N1 + c] = oz;
-      if (REUSE && own) {
-        nbb = bb;
-        nbref = bref;
-        if constexpr (TWO) {
-          curl_b<T>(BXF, BYF, BZF, ncb);
-          nbo[0] = lbx(0, 0, 0);
-          nbo[1] = lby(0, 0, 0);
-          nbo[2] = lbz(0, 0, 0);
-        }
-      }
-    });
+    }
   };
   // cleanEpar of plane q on the tile and its 1-cell halo: each thread its
   // own cell, then the first RING threads the halo ring
-  auto clean_plane = [&](int q) {
-    clean_at(q, tid % TX + 1, tid / TX + 1, true);
+  // (own_tag false: the own cell's values are not needed)
+  auto clean_plane = [&](int q, int qs, auto own_tag, Own& dst) {
+    clean_at(q, qs, tid % TX + 1, tid / TX + 1, own_tag, dst);
     for (int r = tid; r < RING; r += NT) {
       int mx, my;
       if (r < W1) {
@@ -358,22 +379,12 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
         mx = W1 - 1;
         my = r - 2 * W1 - TY + 1;
       }
-      clean_at(q, mx, my, false);
-    }
-  };
-  auto hand_over = [&]() {
-    cbb = nbb;
-    cbref = nbref;
-    if constexpr (TWO) {
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        ccb[m] = ncb[m];
-        cbo[m] = nbo[m];
-      }
+      clean_at(q, qs, mx, my, std::false_type{}, dst);
     }
   };
 
-  // prologue: planes k0-2 .. k0+1 raw, k0-1 and k0 cleaned
+  // prologue: planes k0-2 .. k0+1 raw, k0-1 and k0 cleaned (k0's own-cell
+  // values where the first push reads them)
   load_plane(k0 - 2);
   store_plane(k0 - 2);
   load_plane(k0 - 1);
@@ -381,14 +392,14 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
   load_plane(k0);
   store_plane(k0);
   __syncthreads();
-  clean_plane(k0 - 1);
+  clean_plane(k0 - 1, k0 - 1, std::false_type{}, own0);
   __syncthreads();
   load_plane(k0 + 1);
   store_plane(k0 + 1);
   __syncthreads();
-  clean_plane(k0);
+  clean_plane(k0, k0, std::true_type{}, own_set(std::integral_constant<int, RG::STATIC ? 0 : 1>{}));
   __syncthreads();
-  hand_over();
+  if constexpr (!RG::STATIC) own0 = own1;
   if (k0 < k1) load_plane(k0 + 2);
 
   const int tx = tid % TX, ty = tid / TX;
@@ -401,10 +412,14 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
       (uint32_t)((g.org + min(gi, g.nx - 1) + (int64_t)min(gj, g.ny - 1) * g.px) * (int64_t)sizeof(S));
 
   unsigned long long skipped = 0ull;  // FIX: bit k - k0 for a skipped plane k
-  for (int k = k0; k < k1; ++k) {
+  // Output plane k; PH is k mod 4 when known at compile time (STATIC),
+  // else -1.
+  auto iteration = [&](int k, auto ph) {
+    constexpr int PH = decltype(ph)::value;
+    const int ks = PH >= 0 ? PH : k;
     // raw plane k+2 arrived during the previous iteration (with TWO, other
     // waves may still push plane k-1: it reads no raw slot)
-    store_plane(k + 2);
+    store_plane(ks + 2);
     // the flag slot of plane k+2 (its previous plane was last read before
     // this iteration)
     if (FIX && tid == 0) nflag[(unsigned)(k + 2) & FM] = 0u;
@@ -436,8 +451,9 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
     if (EARLY && active) load_un();
     if (k + 1 < k1) load_plane(k + 3);
     __syncthreads();
-    clean_plane(k + 1);
+    clean_plane(k + 1, ks + 1, std::true_type{}, own_set(std::integral_constant<int, PH >= 0 ? (PH + 1) & 1 : 1>{}));
     __syncthreads();
+    const Own& cur = own_set(std::integral_constant<int, PH >= 0 ? PH & 1 : 0>{});
     bool skip = false;
     if constexpr (FIX) {
       const unsigned f = nflag[(unsigned)(k - 1) & FM] | nflag[(unsigned)k & FM] | nflag[(unsigned)(k + 1) & FM];
@@ -446,8 +462,9 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
     }
     if (active && !skip) {
       if (!EARLY) load_un();
-      const auto lex = lds_cell<S, W1>(cex, N1, k, lc1, slot<2>), ley = lds_cell<S, W1>(cey, N1, k, lc1, slot<2>);
-      const auto lez = lds_cell<S, W1>(cez, N1, k, lc1, slot<3>);
+      const auto lex = lds_cell<S, W1>(cex, N1, ks, lc1, slot<RG::CX>);
+      const auto ley = lds_cell<S, W1>(cey, N1, ks, lc1, slot<RG::CY>);
+      const auto lez = lds_cell<S, W1>(cez, N1, ks, lc1, slot<RG::CZ>);
       auto EX = [&](int a, int b, int d) { return (T)lex(a, b, d); };
       auto EY = [&](int a, int b, int d) { return (T)ley(a, b, d); };
       auto EZ = [&](int a, int b, int d) { return (T)lez(a, b, d); };
@@ -460,22 +477,19 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
       v[1] = (P)ley(0, 0, 0);
       v[2] = (P)lez(0, 0, 0);
       if constexpr (TWO) {
-        rhs_cell_pre<PR>(EX, EY, EZ, RHO, cbb, cbref, A.cc, tt[0], tt[1], tt[2], ccb, d);
-        v[3] = (P)cbo[0];
-        v[4] = (P)cbo[1];
-        v[5] = (P)cbo[2];
+        rhs_cell_pre<PR>(EX, EY, EZ, RHO, cur.bb, cur.bref, A.cc, tt[0], tt[1], tt[2], cur.cb, d);
+        v[3] = (P)cur.bo[0];
+        v[4] = (P)cur.bo[1];
+        v[5] = (P)cur.bo[2];
       } else {
-        const auto lbx = lds_cell<S, W2>(rbx, N2, k, lc2, slot<RG::BX>);
-        const auto lby = lds_cell<S, W2>(rby, N2, k, lc2, slot<RG::BY>);
-        const auto lbz = lds_cell<S, W2>(rbz, N2, k, lc2, slot<RG::BZ>);
+        const auto lbx = lds_cell<S, W2>(rbx, N2, ks, lc2, slot<RG::BX>);
+        const auto lby = lds_cell<S, W2>(rby, N2, ks, lc2, slot<RG::BY>);
+        const auto lbz = lds_cell<S, W2>(rbz, N2, ks, lc2, slot<RG::BZ>);
         auto BXF = [&](int a, int b, int d) { return (T)lbx(a, b, d); };
         auto BYF = [&](int a, int b, int d) { return (T)lby(a, b, d); };
         auto BZF = [&](int a, int b, int d) { return (T)lbz(a, b, d); };
-        if constexpr (REUSE)
-          rhs_cell_bb<PR, PR::null_fp64 && !FIX>(EX, EY, EZ, BXF, BYF, BZF, RHO, cbb, cbref, A.cc, tt[0], tt[1],
-                                                 tt[2], d);
-        else
-          rhs_cell<PR>(EX, EY, EZ, BXF, BYF, BZF, RHO, A.cc, tt[0], tt[1], tt[2], d);
+        rhs_cell_bb<PR, PR::null_fp64 && !FIX>(EX, EY, EZ, BXF, BYF, BZF, RHO, cur.bb, cur.bref, A.cc, tt[0],
+                                               tt[1], tt[2], d);
         v[3] = (P)lbx(0, 0, 0);
         v[4] = (P)lby(0, 0, 0);
         v[5] = (P)lbz(0, 0, 0);
@@ -502,8 +516,19 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
       st_at(ob + cs, oxy, (S)v[4]);
       st_at(ob + 2 * cs, oxy, (S)v[5]);
     }
-    hand_over();
+    if constexpr (PH < 0) own0 = own1;
     if constexpr (!TWO) __syncthreads();  // plane k's rings are free for plane k+3 / k+2
+  };
+  if constexpr (RG::STATIC) {
+    // k0 is a multiple of 4 (z-chunks are), so plane kk + j has phase j
+    for (int kk = k0; kk < k1; kk += 4) {
+      iteration(kk, std::integral_constant<int, 0>{});
+      if (kk + 1 < k1) iteration(kk + 1, std::integral_constant<int, 1>{});
+      if (kk + 2 < k1) iteration(kk + 2, std::integral_constant<int, 2>{});
+      if (kk + 3 < k1) iteration(kk + 3, std::integral_constant<int, 3>{});
+    }
+  } else {
+    for (int k = k0; k < k1; ++k) iteration(k, std::integral_constant<int, -1>{});
   }
   if (FIX && tid == 0) A.fixmask[tile_id] = skipped;
 }
