@@ -29,10 +29,24 @@ coefficients (H1)      G(z)=1+z+z^2/2+z^3/6 per step, three orientations
 div B, static states   div_h B conserved to roundoff; uniform B static
 force-free current     closed-form current for linear E in uniform B;
                        limiter and E.B=0 projection against hand algebra
-user_loop BC           floor copy by construction (ghost == k=0 plane)
-user_uniform BC        closed-form footpoint field at sample points
-user_psr BC            dipole and tanh shape at sample points; blend limits
-                       (s=1 inside -> dipole; s=0 outside -> unchanged)
+user_loop BC           floor copy by construction (ghost == k=0 plane);
+                       driven form: footpoint field at sample points
+user_uniform BC        properties its definition fixes: a 90-degree turn
+                       about the box axis maps Ey onto Ex bit for bit
+                       (radial field, Yee offsets), the tanh(t/t0) ramp by
+                       step ratios, the e^(-r/r0) profile by radius ratios,
+                       linearity in beta, Bz = the initial potential field
+user_psr BC            closed form on uniform fields (tests/closed_forms.py:
+                       radial and tangential relaxation toward the rotated
+                       dipole and the corotation field, each with its own
+                       width and shell), every cell of r < 2R; the widths
+                       or shells swapped fail the same comparison
+absorbing layer,       fixed mode (jg from j, delta capped at 1) and the
+both modes             layer as coded (jg from i, uncapped): closed-form
+                       damping factors of a static uniform field, every
+                       cell and three cells by hand; as coded also
+                       bit-for-bit against oracle/_ref
+cleanEpar guard        fixed mode leaves E unchanged at |B|^2 = 0
 output interpolation   bit-for-bit against the reference's own compiled
 (helpers.F90:93-257)   interpFromEdges / interpFromFaces (oracle/_ref), FP64
                        and FP32, random fields and offsets; linear fields

@@ -225,3 +225,38 @@ def test_errors_are_returned_not_raised_in_c():
         g.set_tuning(0, 65)  # z-chunks above 64 do not fit a tile's plane mask
     with pytest.raises(pytff.TffError):
         g.set_tuning(6, 8)   # no such tile shape
+
+
+def test_psr_parameters_changed_between_graph_runs():
+    """tff_set_bc between two graph runs on the per-procedure path (whose
+    state is always clean): the second run must use the new sphere (a graph
+    captured with the old staging buffer and box would replay stale
+    parameters)."""
+    p = SMALL["psr"]()
+    o, g = _pair(p.n, p.boundary, cc=p.cc, absorb_buff=p.absorb_buff, absorb_fixed=p.absorb_fixed)
+    for t in (o, g):
+        problems.load(t, p)
+        t.preloop()
+    g.set_path(False)
+    o.run(0, 3)
+    g.run(0, 3)
+    params = list(p.bc_params)
+    params[0] = 8.0  # radius 5 -> 8
+    o.set_bc(p.bc_kind, params)
+    g.set_bc(p.bc_kind, params)
+    o.run(3, 3)
+    g.run(3, 3)
+    err = field_errors(g.download, o.field)
+    print("psr radius changed between runs:", err)
+    assert max(err.values()) <= 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_open_box_settled_ghosts_bitwise(precision):
+    """Every axis open and no hook: after one full step the fused path stops
+    carrying and cleaning the ghosts (they no longer change).  Ten steps of
+    captured graphs equal the oracle bit for bit."""
+    p = problems.alfven(24, seed=11, kmax=4, steps=10)
+    p.boundary = (0, 0, 0)
+    o, g = _run_both(p, 10, precision=precision, graph=True)
+    _compare_exact(o, g, dtype=np.float32 if precision == "fp32" else np.float64)

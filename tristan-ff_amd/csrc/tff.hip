@@ -94,7 +94,11 @@ struct tff_handle {
 
   // keyed by the state and clean E buffers and the number of steps captured
   std::map<std::tuple<void*, void*, int>, hipGraphExec_t> graphs;
+  std::vector<hipGraphExec_t> retired;  // invalidated, not yet destroyed
   std::vector<void*> allocs;
+  // staging of tff_output_fields (grown on demand, kept until destroy)
+  void* out_buf = nullptr;
+  size_t out_cap = 0;
 };
 
 namespace {
@@ -137,10 +141,20 @@ S* base_m(const Vec& v) {
   return (S*)v.p[0];
 }
 
+// Stop replaying the captured graphs (buffer roles or hook parameters
+// changed).  They are destroyed later, where the library captures anew or
+// tears down, so that entry points that only clean the state never free.
 void invalidate_graphs(tff_handle* h) {
-  for (auto& kv : h->graphs) hipGraphExecDestroy(kv.second);
+  for (auto& kv : h->graphs) h->retired.push_back(kv.second);
   h->graphs.clear();
   h->ghosts_settled = false;
+}
+
+// destroy the graphs invalidate_graphs set aside (the stream is idle or
+// ordered after their last launch)
+void drop_retired(tff_handle* h) {
+  for (auto e : h->retired) hipGraphExecDestroy(e);
+  h->retired.clear();
 }
 
 int last_launch(tff_handle* h, const char* what) {
@@ -717,13 +731,21 @@ struct DoStep {
     auto key = std::make_tuple(h->es.p[0], h->ce.p[0], count);
     auto it = h->graphs.find(key);
     if (it == h->graphs.end()) {
-      hipGraph_t graph;
+      if (!h->retired.empty()) {
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        drop_retired(h);
+      }
+      hipGraph_t graph = nullptr;
       HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
       for (int s = 0; s < count && rc == TFF_OK; ++s)
         rc = h->fused ? enqueue_fused_step<PR>(h, h->d_step, s) : enqueue_step<PR>(h, h->d_step, s);
       if (rc == TFF_OK) hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(64), 0, h->stream, h->d_step, count);
       hipError_t e = hipStreamEndCapture(h->stream, &graph);
-      if (rc) return rc;
+      if (rc) {
+        // keep the enqueue's own error; the partial graph is not used
+        if (e == hipSuccess && graph) hipGraphDestroy(graph);
+        return rc;
+      }
       if (e != hipSuccess) return fail(h, TFF_EHIP, "hipStreamEndCapture failed");
       hipGraphExec_t exec;
       e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -747,18 +769,23 @@ int output_dims(const tff_handle* h, int istep, int d[3]) {
   return TFF_OK;
 }
 
+// Bytes of device staging the six output datasets of one call take.
+size_t output_bytes(const tff_handle* h, int istep, int host_dtype) {
+  int d[3];
+  output_dims(h, istep, d);
+  return 6 * (size_t)d[0] * d[1] * d[2] * (host_dtype == TFF_HOST_F64 ? 8 : 4);
+}
+
+// The six datasets into the device staging `dbuf` (k_output), then into the
+// caller's host buffers; returns when they are there.
 template <class PR>
 struct DoOutput {
-  static int run(tff_handle* h, int istep, void* const* out, int host_dtype) {
+  static int run(tff_handle* h, int istep, void* const* out, int host_dtype, void* dbuf) {
     using S = typename PR::S;
     int d[3];
     output_dims(h, istep, d);
     const int64_t nout = (int64_t)d[0] * d[1] * d[2];
     const size_t hsz = host_dtype == TFF_HOST_F64 ? 8 : 4;
-    void* dbuf = nullptr;
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (hipMalloc(&dbuf, 6 * (size_t)nout * hsz) != hipSuccess)
-      return fail(h, TFF_ENOMEM, "hipMalloc of the output buffer failed");
     const dim3 grid((unsigned)((d[0] + 63) / 64), (unsigned)((d[1] + 3) / 4), (unsigned)d[2]);
     if (hsz == 8)
       hipLaunchKernelGGL((k_output<S, double>), grid, dim3(256), 0, h->stream, h->g, istep, d[0], d[1], d[2],
@@ -772,7 +799,6 @@ struct DoOutput {
                          h->stream) != hipSuccess)
         rc = fail(h, TFF_EHIP, "hipMemcpyAsync of an output dataset failed");
     if (hipStreamSynchronize(h->stream) != hipSuccess && !rc) rc = fail(h, TFF_EHIP, "output: stream failed");
-    hipFree(dbuf);
     return rc;
   }
 };
@@ -933,7 +959,9 @@ void tff_destroy(tff_handle* h) {
   if (!h) return;
   if (h->stream) hipStreamSynchronize(h->stream);
   invalidate_graphs(h);
+  drop_retired(h);
   for (void* p : h->allocs) hipFree(p);
+  if (h->out_buf) hipFree(h->out_buf);
   if (h->psr_stage) hipFree(h->psr_stage);
   hipFree(h->d_step);
   hipFree(h->d_flag);
@@ -1018,6 +1046,10 @@ int tff_set_bc(tff_handle* h, int kind, const double* params, int nparams) {
   std::memset(h->bc_host, 0, sizeof(h->bc_host));
   for (int t = 0; t < nparams; ++t) h->bc_host[t] = params[t];
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  // captured graphs hold the hook's staging buffer and box: never replay
+  // them with the new parameters (on every path, whatever the kind)
+  invalidate_graphs(h);
+  drop_retired(h);
   HIPCHK(h, hipMemcpyAsync(h->d_bc, h->bc_host, 16 * sizeof(double), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (h->psr_stage) {
@@ -1043,7 +1075,6 @@ int tff_set_bc(tff_handle* h, int kind, const double* params, int nparams) {
       if (hipMalloc(&h->psr_stage, nb) != hipSuccess) return fail(h, TFF_ENOMEM, "psr staging");
     }
   }
-  if (kind != h->bc_kind) invalidate_graphs(h);
   h->bc_kind = kind;
   return TFF_OK;
 }
@@ -1151,7 +1182,16 @@ int tff_output_fields(tff_handle* h, int istep, void* const out[6], int host_dty
     if (!out[f]) return fail(h, TFF_EINVAL, "null output buffer");
   int rc = ensure_clean(h);
   if (rc) return rc;
-  return dispatch<DoOutput>(h, istep, out, host_dtype);
+  const size_t need = output_bytes(h, istep, host_dtype);
+  if (need > h->out_cap) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->out_buf) hipFree(h->out_buf);
+    h->out_buf = nullptr;
+    h->out_cap = 0;
+    if (hipMalloc(&h->out_buf, need) != hipSuccess) return fail(h, TFF_ENOMEM, "hipMalloc of the output buffer failed");
+    h->out_cap = need;
+  }
+  return dispatch<DoOutput>(h, istep, out, host_dtype, h->out_buf);
 }
 
 int tff_reduce(tff_handle* h, double out[4]) {
