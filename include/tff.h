@@ -163,6 +163,19 @@ int tff_run(tff_handle* h, int first_step, int nsteps);
 int tff_output_dims(tff_handle* h, int istep, int dims[3]);
 int tff_output_fields(tff_handle* h, int istep, void* const out[6], int host_dtype);
 
+/* Device staging of tff_output_fields_noalloc: 24 MiB, i.e. six datasets
+ * of up to 1,048,576 nodes as FP32 or 524,288 as FP64. */
+#define TFF_NOALLOC_SCRATCH_BYTES (24 * 1024 * 1024)
+/* tff_output_fields without any device allocation: no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory (tight
+ * output loops, harnesses that forbid hipMalloc).  Same arguments, results
+ * and errors as tff_output_fields; it stages through a fixed device array
+ * of TFF_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends,
+ * and rejects outputs larger than that with TFF_EINVAL.  Every call on one
+ * GPU shares that array: calls that overlap on a GPU (other threads or
+ * handles) race. */
+int tff_output_fields_noalloc(tff_handle* h, int istep, void* const out[6], int host_dtype);
+
 /* FP64 diagnostics over the interior (no reference counterpart):
  * out[0] = 1/2 sum E^2, out[1] = 1/2 sum B^2, out[2] = max |div_h B|,
  * out[3] = sqrt(mean (div_h B)^2). */

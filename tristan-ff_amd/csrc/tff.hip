@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <string>
 #include <type_traits>
@@ -885,6 +886,33 @@ Vec* field_vec(tff_handle* h, int field, int& comp) {
 
 }  // namespace
 
+// ---- staging without allocation ------------------------------------------
+// tff_output_fields_noalloc stages through `scratch`, a file-scope device
+// array that every GPU the library runs on holds (one copy each) until the
+// process ends.  In host code `scratch` names a host-side stand-in, never
+// the device memory: each device's own address is looked up once, with that
+// device current (tff_create), and only that address is used afterwards.
+static __device__ __align__(256) unsigned char scratch[TFF_NOALLOC_SCRATCH_BYTES];
+namespace {
+std::mutex scratch_mu;
+std::map<int, void*> scratch_addr;  // device -> its copy of scratch
+
+int scratch_setup(int device) {
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_addr.count(device)) return TFF_OK;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return TFF_EHIP;
+  scratch_addr[device] = p;
+  return TFF_OK;
+}
+
+void* scratch_of(int device) {
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  auto it = scratch_addr.find(device);
+  return it == scratch_addr.end() ? nullptr : it->second;
+}
+}  // namespace
+
 // ============================================================================
 extern "C" {
 
@@ -913,7 +941,7 @@ int tff_create(const tff_config* cfg, tff_handle** out) {
   h->nalloc = h->g.plane * (cfg->nz + 2 * NG) + al;
   h->nalloc = (h->nalloc + 63) / 64 * 64;  // keep every component 256-byte aligned
   h->g.cs = h->nalloc;
-  if (hipSetDevice(cfg->device) != hipSuccess) {
+  if (hipSetDevice(cfg->device) != hipSuccess || scratch_setup(cfg->device) != TFF_OK) {
     delete h;
     return TFF_EHIP;
   }
@@ -1192,6 +1220,27 @@ int tff_output_fields(tff_handle* h, int istep, void* const out[6], int host_dty
     h->out_cap = need;
   }
   return dispatch<DoOutput>(h, istep, out, host_dtype, h->out_buf);
+}
+int tff_output_fields_noalloc(tff_handle* h, int istep, void* const out[6], int host_dtype) {
+  if (!h || !out) return TFF_EINVAL;
+  if (istep < 2)
+    return fail(h, TFF_EINVAL, "output_istep must be >= 2 (the istep = 1 debug layout is not supported)");
+  if (host_dtype != TFF_HOST_F64 && host_dtype != TFF_HOST_F32) return fail(h, TFF_EINVAL, "unknown host dtype");
+  for (int f = 0; f < 6; ++f)
+    if (!out[f]) return fail(h, TFF_EINVAL, "null output buffer");
+  const size_t need = output_bytes(h, istep, host_dtype);
+  if (need > (size_t)TFF_NOALLOC_SCRATCH_BYTES)
+    return fail(h, TFF_EINVAL,
+                "tff_output_fields_noalloc: the six datasets take " + std::to_string(need) +
+                    " bytes, more than its " + std::to_string(TFF_NOALLOC_SCRATCH_BYTES) +
+                    "-byte scratch (use tff_output_fields, or a larger istep)");
+  void* p = scratch_of(h->cfg.device);
+  if (!p) return fail(h, TFF_ESTATE, "no scratch address for this device");
+  // cleaning the state only launches kernels (invalidated graphs are freed
+  // later, where the library captures anew)
+  int rc = ensure_clean(h);
+  if (rc) return rc;
+  return dispatch<DoOutput>(h, istep, out, host_dtype, p);
 }
 
 int tff_reduce(tff_handle* h, double out[4]) {

@@ -89,6 +89,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.tff_reduce.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
     lib.tff_output_dims.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.tff_output_fields.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
+    lib.tff_output_fields_noalloc.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
     lib.tff_sync.argtypes = [P]
     lib.tff_event_record.argtypes = [P, ctypes.c_int]
     lib.tff_event_elapsed.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
@@ -239,16 +240,18 @@ class Meshblock:
         self._check(self.lib.tff_reduce(self.h, out), "tff_reduce")
         return list(out)
 
-    def output_fields(self, istep: int, dtype=np.float32) -> dict:
+    def output_fields(self, istep: int, dtype=np.float32, noalloc: bool = False) -> dict:
         """writeFields_hdf5's field datasets (output_istep >= 2): ex ey ez
         bx by bz node interpolants at every istep-th node, each shaped
-        (k, j, i)."""
+        (k, j, i).  noalloc: through tff_output_fields_noalloc (no device
+        allocation; outputs up to TFF_NOALLOC_SCRATCH_BYTES)."""
         d = (ctypes.c_int * 3)()
         self._check(self.lib.tff_output_dims(self.h, int(istep), d), "tff_output_dims")
         arrs = [np.empty((d[2], d[1], d[0]), dtype=dtype) for _ in range(6)]
         ptrs = (ctypes.c_void_p * 6)(*[a.ctypes.data for a in arrs])
         dt = 1 if np.dtype(dtype) == np.float32 else 0
-        self._check(self.lib.tff_output_fields(self.h, int(istep), ptrs, dt), "tff_output_fields")
+        fn = "tff_output_fields_noalloc" if noalloc else "tff_output_fields"
+        self._check(getattr(self.lib, fn)(self.h, int(istep), ptrs, dt), fn)
         return dict(zip(("ex", "ey", "ez", "bx", "by", "bz"), arrs))
 
     def sync(self):
