@@ -163,6 +163,16 @@ int tff_run(tff_handle* h, int first_step, int nsteps);
 int tff_output_dims(tff_handle* h, int istep, int dims[3]);
 int tff_output_fields(tff_handle* h, int istep, void* const out[6], int host_dtype);
 
+/* What userOutput(var, ...) of the user files reads at the same nodes
+ * (write_output.F90:375-383): out[0..5] the raw ex ey ez bx by bz at cell
+ * (i, j, k) = (i1, j1, k1) * istep, out[6] the charge density there in
+ * computeRho's form (field_solver.F90:57-59) of the state written.  The
+ * reference's userOutput reads the rho its last rk3Push left (the third
+ * stage's input); this is the rho of the output state, the value of the
+ * call to computeRho that tristanmainloop.F90:92 leaves commented out.
+ * Same layout and dtypes as tff_output_fields. */
+int tff_output_samples(tff_handle* h, int istep, void* const out[7], int host_dtype);
+
 /* Device staging of tff_output_fields_noalloc: 24 MiB, i.e. six datasets
  * of up to 1,048,576 nodes as FP32 or 524,288 as FP64. */
 #define TFF_NOALLOC_SCRATCH_BYTES (24 * 1024 * 1024)
@@ -207,12 +217,31 @@ int tff_set_path(tff_handle* h, int fused);
  * current choice (tile index, its x and y extent, z-chunk). */
 int tff_set_tuning(tff_handle* h, int tile, int kchunk);
 int tff_get_tuning(tff_handle* h, int* tile, int* tx, int* ty, int* kchunk);
+/* Phase timers for a timing report like the reference's makeReport
+ * (tristanmainloop.F90:108-151): while on, every step runs eagerly (no
+ * graph replay) with HIP events around its phases -- TFF_PHASE_SOLVER the
+ * stage kernels (rk3Push, rk3Update, cleanEpar; fld_solver without the
+ * exchange), TFF_PHASE_HOOK the user boundary kernels (usr_funcs),
+ * TFF_PHASE_EXCHANGE the ghost exchange and ghost-cell passes.
+ * tff_phase_times synchronizes, returns the GPU milliseconds of each phase
+ * since the last call (or since timing was switched on) and restarts the
+ * sums. */
+#define TFF_PHASES 3
+#define TFF_PHASE_SOLVER 0
+#define TFF_PHASE_HOOK 1
+#define TFF_PHASE_EXCHANGE 2
+int tff_set_phase_timing(tff_handle* h, int on);
+int tff_phase_times(tff_handle* h, double ms[3]);
+
 /* Device layout of one field component (no reference counterpart: the
  * reference's arrays are host Fortran arrays): out[0] = padded x-row pitch,
  * out[1] = plane pitch, out[2] = offset of interior cell (0,0,0), out[3] =
  * elements per component (nalloc), all in elements; out[4] = bytes per
  * element. */
 int tff_get_layout(tff_handle* h, long long out[5]);
+/* The box's interior size n[0..2] (the deck's mx0, my0, mz0) and the
+ * precision it was created with. */
+int tff_get_size(tff_handle* h, int n[3], int* precision);
 
 /* ---- batched path (parameter scans) ----------------------------------
  * Many independent meshblocks of uneven size advanced together: per RK

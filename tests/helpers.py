@@ -70,3 +70,22 @@ def near_null_cells(f):
     for m in (x0 * x0 + y0 * y0 + z0 * z0, x1 * x1 + y1 * y1 + z1 * z1, x2 * x2 + y2 * y2 + z2 * z2):
         near |= m < 1e-4 * bref
     return near
+
+
+def h5_read(path, name, dtype, tmpdir):
+    """One dataset of an HDF5 file, read with the HDF5 tools' h5dump (no
+    Python HDF5 module here): raw little-endian values, flattened."""
+    import os
+    import shutil
+    import subprocess
+    h5dump = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+    out = os.path.join(str(tmpdir), os.path.basename(str(path)) + "_" + name.replace(":", "_") + ".bin")
+    subprocess.run([h5dump, "-d", name, "-b", "LE", "-o", out, str(path)], check=True, capture_output=True)
+    return np.fromfile(out, dtype=dtype)
+
+
+def h5_header(path):
+    import shutil
+    import subprocess
+    h5dump = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+    return subprocess.run([h5dump, "-H", str(path)], check=True, capture_output=True, text=True).stdout

@@ -46,6 +46,34 @@ def build_lib(verbose: bool = False) -> str:
     return LIB
 
 
+HDF5_ROOT = os.environ.get("HDF5_ROOT", "/opt/conda")
+IO_LIB = os.path.join(OUT, "libtff_io.so")
+
+
+def build_io(verbose: bool = False) -> str:
+    """libtff_io.so: the output files of writeOutput through the serial HDF5
+    C API (host C, linked against libtff.so and libhdf5).  Skipped, with a
+    message, when no HDF5 headers are found."""
+    inc = os.path.join(HDF5_ROOT, "include")
+    lib = os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"hdf5.h not found under {inc}: libtff_io.so not built", flush=True)
+        return ""
+    src = os.path.join(PKG, "csrc", "tff_io.c")
+    tmp = IO_LIB + ".tmp"
+    # RUNPATH (not RPATH): the HDF5 directory applies to this library's own
+    # dependencies only, never to the HIP runtime's C++ library
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-ffp-contract=off",
+           "-I", os.path.join(ROOT, "include"), "-I", inc, src, "-o", tmp,
+           "-L", OUT, "-ltff", "-L", lib, "-lhdf5", "-lm",
+           "-Wl,--enable-new-dtags", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{lib}"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
+
+
 def build_fortran(verbose: bool = False) -> str:
     """Fortran host (amdflang): deck parser, main loop and user files that
     call libtff.so through ISO_C_BINDING."""
@@ -63,6 +91,7 @@ def main(argv=None) -> int:
     ap.add_argument("-v", "--verbose", action="store_true")
     a = ap.parse_args(argv)
     print(build_lib(a.verbose))
+    print(build_io(a.verbose))
     if a.fortran and shutil.which("amdflang"):
         print(build_fortran(a.verbose))
     return 0

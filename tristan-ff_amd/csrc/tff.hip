@@ -74,6 +74,13 @@ struct tff_handle {
   int red_blocks = 0;
 
   hipEvent_t ev[16] = {};
+  // phase timers (tff_set_phase_timing): event pairs per phase, recorded
+  // around the solver kernels, the user hook and the ghost exchange
+  bool phases = false;
+  std::vector<hipEvent_t> ppool;
+  std::vector<int> pphase;  // phase of each recorded pair
+  size_t pused = 0;
+  double psum_ms[TFF_PHASES] = {0.0};
   bool timing = false;
   std::vector<hipEvent_t> tpool;
   size_t tused = 0;
@@ -103,6 +110,32 @@ struct tff_handle {
 };
 
 namespace {
+
+// Events around one phase of a step while phase timing is on (nothing
+// otherwise); the pairs are read and summed by tff_phase_times.
+struct PhaseScope {
+  tff_handle* h;
+  hipEvent_t e1 = nullptr;
+  PhaseScope(tff_handle* hh, int phase) : h(hh) {
+    if (!h->phases) return;
+    while (h->ppool.size() < h->pused + 2) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) {
+        h->phases = false;
+        return;
+      }
+      h->ppool.push_back(e);
+    }
+    hipEventRecord(h->ppool[h->pused], h->stream);
+    e1 = h->ppool[h->pused + 1];
+    h->pphase.resize(h->pused / 2 + 1);
+    h->pphase[h->pused / 2] = phase;
+    h->pused += 2;
+  }
+  ~PhaseScope() {
+    if (e1) hipEventRecord(e1, h->stream);
+  }
+};
 
 int fail(tff_handle* h, int code, const std::string& msg) {
   if (h) h->err = msg;
@@ -174,6 +207,7 @@ template <class PR>
 int launch_rk3(tff_handle* h, const Vec& xe, const Vec& xb, const Vec& ue, const Vec& ub,
                const Vec& ye, const Vec& yb, double c1, double c2, double c3) {
   using S = typename PR::S;
+  PhaseScope phase(h, TFF_PHASE_SOLVER);
   using P = typename PR::P;
   Absorb<P> ab{};
   ab.on = h->cfg.boundary[0] == TFF_ABSORB;
@@ -214,6 +248,7 @@ int launch_rk3(tff_handle* h, const Vec& xe, const Vec& xb, const Vec& ue, const
 template <class PR>
 int launch_clean(tff_handle* h, const Vec& xe, const Vec& xb, const Vec& ye) {
   using S = typename PR::S;
+  PhaseScope phase(h, TFF_PHASE_SOLVER);
   hipLaunchKernelGGL(k_clean<PR>, box_grid(h), dim3(BX, BY), 0, h->stream, h->g, cv3<S>(xe),
                      cv3<S>(xb), mv3<S>(ye), h->cfg.absorb_fixed);
   return last_launch(h, "k_clean");
@@ -223,6 +258,7 @@ template <class S>
 int launch_fill(tff_handle* h, std::initializer_list<const Vec*> vecs) {
   const int* b = h->cfg.boundary;
   if (b[0] != TFF_PERIODIC && b[1] != TFF_PERIODIC && b[2] != TFF_PERIODIC) return TFF_OK;
+  PhaseScope phase(h, TFF_PHASE_EXCHANGE);
   FillSet<S> fs{};
   fs.n = 0;
   for (const Vec* v : vecs)
@@ -241,6 +277,8 @@ template <class PR>
 int launch_bc(tff_handle* h, const Vec& e, const Vec& b, const int* step_ptr, int step_val) {
   using S = typename PR::S;
   using P = typename PR::P;
+  if (h->bc_kind == TFF_BC_NONE) return TFF_OK;
+  PhaseScope phase(h, TFF_PHASE_HOOK);
   const int sx = h->g.nx + 2 * NG, sy = h->g.ny + 2 * NG;
   const int tb = 256;
   const P cc = std::is_same<S, P>::value ? (P)(S)h->cfg.cc : (P)h->cfg.cc;
@@ -454,6 +492,7 @@ int launch_fix_t(tff_handle* h, const StageArgs<typename PR::S, typename PR::P>&
 
 template <class PR>
 int launch_stage(tff_handle* h, const StageArgs<typename PR::S, typename PR::P>& a, int tile) {
+  PhaseScope phase(h, TFF_PHASE_SOLVER);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (h->timing) {
     while (h->tpool.size() < h->tused + 2) {
@@ -536,13 +575,18 @@ int ghost_passes(tff_handle* h, const Vec& ge_in, const Vec& prev_b, const Vec& 
                      (h->bc_kind == TFF_BC_LOOP && h->bc_host[8] == 2.0);
   const int3 per = blind ? int3{bd[0] == TFF_PERIODIC, bd[1] == TFF_PERIODIC, bd[2] == TFF_PERIODIC}
                          : int3{0, 0, 0};
-  hipLaunchKernelGGL(k_carry<S>, dim3(nb), dim3(tb), 0, h->stream, h->g, cv3<S>(ge_in), cv3<S>(prev_b),
-                     mv3<S>(oe), mv3<S>(ob), n, per);
-  int rc = last_launch(h, "k_carry");
+  int rc;
+  {
+    PhaseScope phase(h, TFF_PHASE_EXCHANGE);
+    hipLaunchKernelGGL(k_carry<S>, dim3(nb), dim3(tb), 0, h->stream, h->g, cv3<S>(ge_in), cv3<S>(prev_b),
+                       mv3<S>(oe), mv3<S>(ob), n, per);
+    rc = last_launch(h, "k_carry");
+  }
   if (rc) return rc;
   if ((rc = launch_bc<PR>(h, oe, ob, step_ptr, step_val))) return rc;
   if ((rc = launch_fill<S>(h, {&oe, &ob}))) return rc;
   const int* b = h->cfg.boundary;
+  PhaseScope phase(h, TFF_PHASE_EXCHANGE);
   hipLaunchKernelGGL(k_ghost_clean<PR>, dim3(nb), dim3(tb), 0, h->stream, h->g, b[0] == TFF_PERIODIC,
                      b[1] == TFF_PERIODIC, b[2] == TFF_PERIODIC, cv3<S>(oe), cv3<S>(ob), mv3<S>(h->ge),
                      h->cfg.absorb_fixed, n);
@@ -711,7 +755,7 @@ struct DoStep {
       if ((rc = autotune_t<PR>(h))) return rc;
       // the first step after the state was (re)made clean runs eagerly; the
       // steady state has fixed buffer roles and is replayed from one graph
-      if (!use_graph || h->timing || h->state_clean) {
+      if (!use_graph || h->timing || h->phases || h->state_clean) {
         if ((rc = enqueue_fused_step<PR>(h, nullptr, step))) return rc;
         // this step carried the ghosts through w1, w2 and the state
         if (ghosts_static(h)) h->ghosts_settled = true;
@@ -720,7 +764,7 @@ struct DoStep {
       }
     } else {
       if ((rc = ensure_clean_t<PR>(h))) return rc;
-      if (!use_graph || h->timing) {
+      if (!use_graph || h->timing || h->phases) {
         rc = enqueue_step<PR>(h, nullptr, step);
         if (rc) return rc;
         std::swap(h->es, h->ce);
@@ -769,6 +813,33 @@ int output_dims(const tff_handle* h, int istep, int d[3]) {
   d[2] = (h->g.nz + istep - 1) / istep;
   return TFF_OK;
 }
+
+// The raw components and rho at the output nodes (k_output_samples) into
+// the caller's seven host buffers, staged through `dbuf`.
+template <class PR>
+struct DoSamples {
+  static int run(tff_handle* h, int istep, void* const* out, int host_dtype, void* dbuf) {
+    using S = typename PR::S;
+    int d[3];
+    output_dims(h, istep, d);
+    const int64_t nout = (int64_t)d[0] * d[1] * d[2];
+    const size_t hsz = host_dtype == TFF_HOST_F64 ? 8 : 4;
+    const dim3 grid((unsigned)((d[0] + 63) / 64), (unsigned)((d[1] + 3) / 4), (unsigned)d[2]);
+    if (hsz == 8)
+      hipLaunchKernelGGL((k_output_samples<S, double>), grid, dim3(256), 0, h->stream, h->g, istep, d[0], d[1],
+                         d[2], cv3<S>(h->es), cv3<S>(h->bs), (double*)dbuf);
+    else
+      hipLaunchKernelGGL((k_output_samples<S, float>), grid, dim3(256), 0, h->stream, h->g, istep, d[0], d[1],
+                         d[2], cv3<S>(h->es), cv3<S>(h->bs), (float*)dbuf);
+    int rc = last_launch(h, "k_output_samples");
+    for (int f = 0; f < 7 && !rc; ++f)
+      if (hipMemcpyAsync(out[f], (char*)dbuf + (size_t)f * nout * hsz, (size_t)nout * hsz, hipMemcpyDeviceToHost,
+                         h->stream) != hipSuccess)
+        rc = fail(h, TFF_EHIP, "hipMemcpyAsync of an output sample set failed");
+    if (hipStreamSynchronize(h->stream) != hipSuccess && !rc) rc = fail(h, TFF_EHIP, "output: stream failed");
+    return rc;
+  }
+};
 
 // Bytes of device staging the six output datasets of one call take.
 size_t output_bytes(const tff_handle* h, int istep, int host_dtype) {
@@ -1000,6 +1071,7 @@ void tff_destroy(tff_handle* h) {
   for (auto e : h->ev)
     if (e) hipEventDestroy(e);
   for (auto e : h->tpool) hipEventDestroy(e);
+  for (auto e : h->ppool) hipEventDestroy(e);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
 }
@@ -1221,6 +1293,26 @@ int tff_output_fields(tff_handle* h, int istep, void* const out[6], int host_dty
   }
   return dispatch<DoOutput>(h, istep, out, host_dtype, h->out_buf);
 }
+int tff_output_samples(tff_handle* h, int istep, void* const out[7], int host_dtype) {
+  if (!h || !out) return TFF_EINVAL;
+  if (istep < 2)
+    return fail(h, TFF_EINVAL, "output_istep must be >= 2 (the istep = 1 debug layout is not supported)");
+  if (host_dtype != TFF_HOST_F64 && host_dtype != TFF_HOST_F32) return fail(h, TFF_EINVAL, "unknown host dtype");
+  for (int f = 0; f < 7; ++f)
+    if (!out[f]) return fail(h, TFF_EINVAL, "null output buffer");
+  int rc = ensure_clean(h);
+  if (rc) return rc;
+  const size_t need = output_bytes(h, istep, host_dtype) / 6 * 7;
+  if (need > h->out_cap) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->out_buf) hipFree(h->out_buf);
+    h->out_buf = nullptr;
+    h->out_cap = 0;
+    if (hipMalloc(&h->out_buf, need) != hipSuccess) return fail(h, TFF_ENOMEM, "hipMalloc of the output buffer failed");
+    h->out_cap = need;
+  }
+  return dispatch<DoSamples>(h, istep, out, host_dtype, h->out_buf);
+}
 int tff_output_fields_noalloc(tff_handle* h, int istep, void* const out[6], int host_dtype) {
   if (!h || !out) return TFF_EINVAL;
   if (istep < 2)
@@ -1315,6 +1407,14 @@ int tff_get_tuning(tff_handle* h, int* tile, int* tx, int* ty, int* kchunk) {
   return TFF_OK;
 }
 
+int tff_get_size(tff_handle* h, int n[3], int* precision) {
+  if (!h || !n) return TFF_EINVAL;
+  n[0] = h->g.nx;
+  n[1] = h->g.ny;
+  n[2] = h->g.nz;
+  if (precision) *precision = h->cfg.precision;
+  return TFF_OK;
+}
 int tff_get_layout(tff_handle* h, long long out[5]) {
   if (!h || !out) return TFF_EINVAL;
   out[0] = h->g.px;
@@ -1325,6 +1425,29 @@ int tff_get_layout(tff_handle* h, long long out[5]) {
   return TFF_OK;
 }
 
+int tff_set_phase_timing(tff_handle* h, int on) {
+  if (!h) return TFF_EINVAL;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->phases = on != 0;
+  h->pused = 0;
+  for (double& v : h->psum_ms) v = 0.0;
+  return TFF_OK;
+}
+int tff_phase_times(tff_handle* h, double ms[TFF_PHASES]) {
+  if (!h || !ms) return TFF_EINVAL;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (size_t t = 0; t + 1 < h->pused; t += 2) {
+    float m = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&m, h->ppool[t], h->ppool[t + 1]));
+    h->psum_ms[h->pphase[t / 2]] += m;
+  }
+  h->pused = 0;
+  for (int p = 0; p < TFF_PHASES; ++p) {
+    ms[p] = h->psum_ms[p];
+    h->psum_ms[p] = 0.0;
+  }
+  return TFF_OK;
+}
 int tff_stage_kernel_time(tff_handle* h, double* total_ms, long* launches) {
   if (!h || !total_ms || !launches) return TFF_EINVAL;
   HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -124,4 +124,29 @@ __global__ void __launch_bounds__(256) k_output(Geom g, int istep, int ni, int n
   }
 }
 
+// The values the user files' userOutput reads at the output nodes
+// (write_output.F90:375-383, e.g. user_uniform.F90:151-173): the raw
+// components ex..bz at cell (i1, j1, k1) * istep, and the charge density
+// there (computeRho's difference form, field_solver.F90:57-59, of the
+// state being written) as the seventh dataset.
+template <class S, class O>
+__global__ void __launch_bounds__(256) k_output_samples(Geom g, int istep, int ni, int nj, int nk,
+                                                        V3<const S> e, V3<const S> b, O* out) {
+  const int i1 = blockIdx.x * 64 + threadIdx.x % 64;
+  const int j1 = blockIdx.y * 4 + threadIdx.x / 64;
+  const int k1 = blockIdx.z;
+  if (i1 >= ni || j1 >= nj || k1 >= nk) return;
+  const int64_t c = g.at(i1 * istep, j1 * istep, k1 * istep);
+  const int64_t sy = g.px, sz = g.plane;
+  const int64_t nout = (int64_t)ni * nj * nk;
+  const int64_t t = i1 + (int64_t)ni * (j1 + (int64_t)nj * k1);
+  out[t] = (O)e.x[c];
+  out[nout + t] = (O)e.y[c];
+  out[2 * nout + t] = (O)e.z[c];
+  out[3 * nout + t] = (O)b.x[c];
+  out[4 * nout + t] = (O)b.y[c];
+  out[5 * nout + t] = (O)b.z[c];
+  out[6 * nout + t] = (O)((e.x[c] - e.x[c - 1]) + (e.y[c] - e.y[c - sy]) + (e.z[c] - e.z[c - sz]));
+}
+
 }  // namespace tff

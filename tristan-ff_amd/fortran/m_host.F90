@@ -16,10 +16,18 @@ module m_globalnamespace
   real(dprec) :: CC, CCINV
   integer :: final_timestep
   integer :: output_start = 0, output_interval = 10, output_dump = 0
-  ! <output> istep (initialize.F90:179): node stride of the flds.tot datasets;
-  ! below 2 (this driver's default) they are not written
-  integer :: output_istep = 0
+  ! <output> istep (initialize.F90:179, default 4): node stride of the
+  ! flds.tot datasets
+  integer :: output_istep = 4
+  ! output files written so far (write_output.F90:41 output_index)
+  integer :: output_index = 0
+  ! <output> report: the timing report every this many steps (the
+  ! reference's makeReport after every step: 1; 0 = none, steps replayed
+  ! from captured graphs)
+  integer :: report_interval = 1
   integer :: precision_mode = 0, fused_step = 1
+  ! what var1..var3 of flds.tot hold (TFF_USEROUT_*): set by the user file
+  integer :: user_output_kind = 0
   character(len=STR_MAX) :: input_file_name = 'input', output_dir_name = 'output'
 end module m_globalnamespace
 
@@ -59,11 +67,21 @@ module m_readinput
   use m_globalnamespace
   use m_errors
   implicit none
+  ! The registry of every deck parameter read (src/tools/auxiliary.F90:22-40,
+  ! filled by read_input.F90's getInput): echoed by printParams and written
+  ! to params.NNNNN.  Types 1 integer, 2 real (default REAL), 3 logical.
+  integer, parameter :: MAX_PARAMS = 1000
+  integer :: param_count = 0
+  integer :: param_type(MAX_PARAMS)
+  character(len=3) :: param_group(MAX_PARAMS)
+  character(len=STR_MAX) :: param_name(MAX_PARAMS)
+  integer :: param_int(MAX_PARAMS)
+  real :: param_real(MAX_PARAMS)
   interface getInput
     module procedure getIntInput
     module procedure getRealInput
   end interface getInput
-  private :: getIntInput, getRealInput, parseInput
+  private :: getIntInput, getRealInput, parseInput, registerParam, lookup
 contains
   ! src/io/read_input.F90:32-54
   subroutine readCommandlineArgs()
@@ -82,9 +100,13 @@ contains
     end do
   end subroutine readCommandlineArgs
 
-  ! Deck semantics of read_input.F90:56-102: find the line `<blockname>`,
-  ! then, before the next line that starts with '<', the line whose first
-  ! token is varname; the value is the text between '=' and '#'.
+  ! Deck semantics of read_input.F90:56-102.  Records are read one at a
+  ! time; a record's first token is what a list-directed read of a
+  ! character variable gives: the text up to the first blank, comma or
+  ! slash ('=' does not end it).  After the record `<blockname>`, the
+  ! record whose token is varname gives the value (the text between '=' and
+  ! '#', left-adjusted); the search ends at the next record starting with
+  ! '<'.  Blank records are skipped, as a list-directed read skips them.
   function parseInput(blockname, varname, found) result(val)
     character(len=*), intent(in) :: blockname, varname
     logical, intent(out) :: found
@@ -99,7 +121,7 @@ contains
       if (ios /= 0) exit
       line = adjustl(line)
       if (len_trim(line) == 0) cycle
-      p = scan(line, ' =,/'//achar(9))
+      p = scan(line, ' ,/'//achar(9))
       if (p == 0) then
         tok = line
       else
@@ -122,49 +144,158 @@ contains
     close (u)
   end function parseInput
 
-  subroutine getIntInput(blockname, varname, val, def_val)
+  ! read_input.F90:104-120: the registry's group names
+  function simplifyBlockname(blockname) result(bname)
+    character(len=*), intent(in) :: blockname
+    character(len=3) :: bname
+    select case (trim(blockname))
+    case ('node_configuration')
+      bname = 'cpu'
+    case ('grid')
+      bname = 'grd'
+    case ('restart')
+      bname = 'rst'
+    case ('problem')
+      bname = 'prb'
+    case default
+      bname = blockname(1:min(3, len(blockname)))
+    end select
+  end function simplifyBlockname
+
+  integer function lookup(blockname, varname)
+    character(len=*), intent(in) :: blockname, varname
+    integer :: n
+    lookup = 0
+    do n = 1, param_count
+      if (param_group(n) == simplifyBlockname(blockname) .and. trim(param_name(n)) == trim(varname)) then
+        lookup = n
+        return
+      end if
+    end do
+  end function lookup
+
+  ! read_input.F90:122-136 and the registration of each getInput: the first
+  ! reading of a (block, name) pair is kept
+  subroutine registerParam(blockname, varname, t, ival, rval)
+    character(len=*), intent(in) :: blockname, varname
+    integer, intent(in) :: t, ival
+    real, intent(in) :: rval
+    if (lookup(blockname, varname) /= 0) return
+    if (param_count >= MAX_PARAMS) call throwError('ERROR: too many input parameters')
+    param_count = param_count + 1
+    param_type(param_count) = t
+    param_group(param_count) = simplifyBlockname(blockname)
+    param_name(param_count) = varname
+    param_int(param_count) = ival
+    param_real(param_count) = rval
+  end subroutine registerParam
+
+  ! the value text of a key, or the reference's errors (read_input.F90:148-166)
+  subroutine valueOf(blockname, varname, has_default, s, found)
+    character(len=*), intent(in) :: blockname, varname
+    logical, intent(in) :: has_default
+    character(len=STR_MAX), intent(out) :: s
+    logical, intent(out) :: found
+    s = parseInput(blockname, varname, found)
+    if (.not. found .and. .not. has_default) &
+      call throwError('ERROR variable `'//trim(varname)//'` not found in <'//trim(blockname)//'>'// &
+                      new_line('a')//'-- no default value provided')
+  end subroutine valueOf
+
+  subroutine conversionError(blockname, varname, s)
+    character(len=*), intent(in) :: blockname, varname, s
+    call throwError('ERROR variable `'//trim(varname)//'`='//trim(s)//' from <'//trim(blockname)// &
+                    '> not converted'//new_line('a')//'-- no default value provided')
+  end subroutine conversionError
+
+  ! getInt4Input (read_input.F90:138-186): integers are read through
+  ! real(8) and truncated; `register` = .false. reads a key of this driver
+  ! that the reference does not have, kept out of the registry
+  subroutine getIntInput(blockname, varname, val, def_val, register)
     character(len=*), intent(in) :: blockname, varname
     integer, intent(out) :: val
     integer, optional, intent(in) :: def_val
+    logical, optional, intent(in) :: register
     character(len=STR_MAX) :: s
     logical :: found
     real(dprec) :: r
     integer :: ios
-    s = parseInput(blockname, varname, found)
+    call valueOf(blockname, varname, present(def_val), s, found)
     if (found) then
-      read (s, *, iostat=ios) r  ! integers are read through real(8)
+      read (s, *, iostat=ios) r
       if (ios == 0) then
         val = int(r)
-        return
+      else if (present(def_val)) then
+        val = def_val
+      else
+        call conversionError(blockname, varname, s)
       end if
-    end if
-    if (present(def_val)) then
-      val = def_val
     else
-      call throwError('ERROR variable `'//trim(varname)//'` not found in <'//trim(blockname)// &
-                      '> -- no default value provided')
+      val = def_val
     end if
+    if (present(register)) then
+      if (.not. register) return
+    end if
+    call registerParam(blockname, varname, 1, val, 0.0)
   end subroutine getIntInput
 
-  subroutine getRealInput(blockname, varname, val, def_val)
+  ! getReal4Input (read_input.F90:250-298); the registry keeps default REAL
+  subroutine getRealInput(blockname, varname, val, def_val, register)
     character(len=*), intent(in) :: blockname, varname
     real(dprec), intent(out) :: val
     real(dprec), optional, intent(in) :: def_val
+    logical, optional, intent(in) :: register
     character(len=STR_MAX) :: s
     logical :: found
     integer :: ios
-    s = parseInput(blockname, varname, found)
+    call valueOf(blockname, varname, present(def_val), s, found)
     if (found) then
       read (s, *, iostat=ios) val
-      if (ios == 0) return
-    end if
-    if (present(def_val)) then
-      val = def_val
+      if (ios /= 0) then
+        if (present(def_val)) then
+          val = def_val
+        else
+          call conversionError(blockname, varname, s)
+        end if
+      end if
     else
-      call throwError('ERROR variable `'//trim(varname)//'` not found in <'//trim(blockname)// &
-                      '> -- no default value provided')
+      val = def_val
     end if
+    if (present(register)) then
+      if (.not. register) return
+    end if
+    call registerParam(blockname, varname, 2, 0, real(val))
   end subroutine getRealInput
+
+  ! src/tools/auxiliary.F90:72-89 getFMTForReal (width 10)
+  function realFormat(v) result(fmt)
+    real, intent(in) :: v
+    character(len=16) :: fmt
+    if ((abs(v) >= 100000) .or. ((abs(v) < 1e-4) .and. (v /= 0.0))) then
+      fmt = 'ES10.2'
+    else
+      fmt = 'F10.2'
+    end if
+  end function realFormat
+
+  ! initialize.F90:74-107: the registry, one line per parameter
+  subroutine printParams()
+    integer :: n
+    character(len=64) :: fmt
+    write (*, '(A)') '== Simulation parameters ==============================================='
+    do n = 1, param_count
+      select case (param_type(n))
+      case (1)
+        write (*, '(A30,A1,A20,A1,I10)') trim(param_group(n)), ':', trim(param_name(n)), ':', param_int(n)
+      case (2)
+        fmt = '(A30,A1,A20,A1,'//trim(realFormat(param_real(n)))//')'
+        write (*, fmt) trim(param_group(n)), ':', trim(param_name(n)), ':', param_real(n)
+      case default
+        write (*, '(A30,A1,A20,A1,L10)') trim(param_group(n)), ':', trim(param_name(n)), ':', param_int(n) /= 0
+      end select
+    end do
+    write (*, '(A)') '........................................................................'
+  end subroutine printParams
 end module m_readinput
 
 module m_domain

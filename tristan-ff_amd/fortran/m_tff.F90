@@ -121,9 +121,55 @@ module m_tff
       import :: c_int, c_ptr
       type(c_ptr), value :: h
     end function tff_sync
+
+    integer(c_int) function tff_set_phase_timing(h, on) bind(C, name='tff_set_phase_timing')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: on
+    end function tff_set_phase_timing
+
+    integer(c_int) function tff_phase_times(h, ms) bind(C, name='tff_phase_times')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: ms(3)
+    end function tff_phase_times
+
+    ! include/tff_io.h (libtff_io.so)
+    integer(c_int) function tff_io_write_output(h, dir, idx, timestep, istep, user_output, n, names, types, &
+                                                ivals, rvals) bind(C, name='tff_io_write_output')
+      import :: c_int, c_ptr, c_char, c_float
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: dir(*)
+      integer(c_int), value :: idx, timestep, istep, user_output, n
+      character(kind=c_char), intent(in) :: names(*)
+      integer(c_int), intent(in) :: types(*), ivals(*)
+      real(c_float), intent(in) :: rvals(*)
+    end function tff_io_write_output
+
+    type(c_ptr) function tff_io_last_error() bind(C, name='tff_io_last_error')
+      import :: c_ptr
+    end function tff_io_last_error
   end interface
 
+  integer(c_int), parameter :: TFF_PHASE_SOLVER = 0, TFF_PHASE_HOOK = 1, TFF_PHASE_EXCHANGE = 2
+  integer(c_int), parameter :: TFF_USEROUT_NONE = 0, TFF_USEROUT_RHO = 1, TFF_USEROUT_UNIFORM = 2
+
 contains
+  ! text of a C string as a Fortran string
+  function c_text(p) result(msg)
+    type(c_ptr), intent(in) :: p
+    character(len=256) :: msg
+    character(kind=c_char), pointer :: s(:)
+    integer :: i
+    msg = ''
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, s, [256])
+    do i = 1, 256
+      if (s(i) == c_null_char) exit
+      msg(i:i) = s(i)
+    end do
+  end function c_text
+
   ! text of the handle's last error as a Fortran string
   function tff_error_text(h) result(msg)
     type(c_ptr), intent(in) :: h

@@ -16,6 +16,7 @@ module m_userfile
   integer, private :: twist
 contains
   subroutine userInitialize()
+    user_output_kind = TFF_USEROUT_RHO  ! userOutput, user_loop.F90:168-180
     call userReadInput()
     call userInitFields()
     call tffCheck(tff_set_bc(tff_h, TFF_BC_LOOP, [xc, yc1, yc2, zc, bnorm, r0, beta_twist, t0, &
@@ -26,10 +27,12 @@ contains
     call getInput('problem', 'L0', L0)
     call getInput('problem', 'a_L0', a0)
     call getInput('problem', 'h_L0', h0)
-    call getInput('problem', 'r_L0', r0, 0.5d0)
-    call getInput('problem', 't0', t0, 100.d0)
-    call getInput('problem', 'beta_twist', beta_twist, 0.d0)
-    call getInput('problem', 'twist', twist, 0)
+    ! user_loop.F90:29-34: no defaults (the shipped inputs/input.loop lacks
+    ! t0 and beta_twist, so the reference stops there, and so does this)
+    call getInput('problem', 'r_L0', r0)
+    call getInput('problem', 't0', t0)
+    call getInput('problem', 'beta_twist', beta_twist)
+    call getInput('problem', 'twist', twist, 0, register=.false.)
     xc = 0.5d0 * global_mesh%sx
     yc1 = 0.5d0 * global_mesh%sy - a0 * L0
     yc2 = 0.5d0 * global_mesh%sy + a0 * L0

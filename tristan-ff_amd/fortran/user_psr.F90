@@ -13,6 +13,7 @@ module m_userfile
   real(dprec), private :: xc_g, yc_g, zc_g, psr_angle, psr_period, psr_omega, psr_radius
 contains
   subroutine userInitialize()
+    user_output_kind = TFF_USEROUT_RHO  ! userOutput, user_psr.F90:386-398
     call userReadInput()
     call userInitFields()
     call tffCheck(tff_set_bc(tff_h, TFF_BC_PSR, [psr_radius, psr_angle, psr_omega, xc_g, yc_g, &

@@ -12,6 +12,7 @@ module m_userfile
   real(dprec), private :: L0, r0, beta_twist, xc, yc, bnorm, t0
 contains
   subroutine userInitialize()
+    user_output_kind = TFF_USEROUT_UNIFORM  ! userOutput, user_uniform.F90:151-173
     call userReadInput()
     call userInitFields()
     call tffCheck(tff_set_bc(tff_h, TFF_BC_UNIFORM, [xc, yc, r0, beta_twist, t0, bnorm], 6), &
