@@ -1,11 +1,16 @@
 #!/usr/bin/env python3
 """Benchmark of the force-free step on one MI355X.
 
-Workload: uniform B0 = z^ plus a seeded Alfvenic perturbation (|dB|/B0 = 0.05,
-1 <= |k| <= 8), 256^3 periodic Yee grid, CC = 0.45; each timed step is one
-full SSP-RK3 timestep (three stages, each with the force-free current, the
-update, the E.B = 0 projection and the ghost fill), inputs resident in HBM.
-Both paths run the same steps from the same initial state:
+Headline workload: uniform B0 = z^ plus a seeded Alfvenic perturbation
+(|dB|/B0 = 0.05, 1 <= |k| <= 8), 256^3 periodic Yee grid, CC = 0.45; each
+timed step is one full SSP-RK3 timestep (three stages, each with the
+force-free current, the update, the E.B = 0 projection and the ghost fill),
+inputs resident in HBM.  A second workload is measured the same way and
+reported in the same line: the aligned rotator (user_psr form) at 512^3, a
+dipole of radius 20 cells, light cylinder at 80 cells, a 40-cell fixed
+absorbing layer on every side and the conducting-sphere hook every stage.
+Initial conditions are generated on the device.  Both paths run the same
+steps from the same initial state:
 
 * fp32_bulk -- FP32 storage and arithmetic (fused multiply-adds, the
   scale-exact stage update), FP64 redo near magnetic nulls, FP64
@@ -15,7 +20,16 @@ Both paths run the same steps from the same initial state:
 The FP32-bulk energy is compared with the FP64 path's energy after the run.
 ``roofline`` is for the dominant kernel (the stage kernel of the FP32-bulk
 path): algorithmic bytes per launch over its average launch time, the time
-taken with HIP events on the library's own stream.
+taken with HIP events on the library's own stream; ``traffic`` is the
+calibrated FETCH_SIZE + WRITE_SIZE per launch stored under profiles/ by
+traffic.py, reported only when it was measured on the kernel sources being
+run (their hash is compared).
+
+``cpu_baseline`` is the reference's own field_solver.F90 (oracle/_ref) on
+the host: one process on one core, and the aggregate of eight concurrent
+single-core processes (the reference's parallelism is MPI ranks, one core
+each), each on a bounded sample of the same workload.  It runs before the
+GPU is touched.
 """
 from __future__ import annotations
 
@@ -23,6 +37,7 @@ import argparse
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -96,86 +111,101 @@ def run_path(precision: str, prob, steps: int, warmup: int, timing_steps: int, t
     }
 
 
-def cpu_baseline(n: int, budget_s: float):
-    """The reference's own field_solver.F90 (oracle/_ref, FP32 as shipped,
-    amdflang -O3) on one host core: whole timesteps of the same workload
-    until about `budget_s` seconds of CPU work have been timed."""
-    try:
-        import numpy as np
-        from oracle.oracle import RefSolver
-        import problems
-        if not RefSolver.available(np.float32):
-            raise FileNotFoundError("oracle/_ref not built")
-        p = problems.alfven(n, seed=1, kmax=8)
-        r = RefSolver(p.n, boundary=p.boundary, cc=p.cc, dtype=np.float32)
-        for f, a in p.fields.items():
-            r.put(f, a.astype(np.float32))
-        r.exchange()
-        t = time.perf_counter()
-        r.run(1)
-        el = time.perf_counter() - t
-        nsteps = 1
-        more = min(29, int(budget_s / max(el, 1e-9)) - 1)
-        if more > 0:
-            t = time.perf_counter()
-            r.run(more)
-            el += time.perf_counter() - t
-            nsteps += more
-        return {"value": p.cells * nsteps / el, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
-                "sample": f"{nsteps} timestep(s) of the Alfvenic workload at {n}^3, reference "
-                          f"field_solver.F90 built FP32 as shipped (amdflang -O3), "
-                          f"single rank, serial periodic ghost fill; {el:.1f} s"}
-    except Exception as e:  # the baseline is reported, never required
-        return {"value": None, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
-                "sample": f"unavailable: {e}"}
-
-
-def load_traffic(workload: str):
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
-    if os.path.exists(path):
-        with open(path) as fh:
-            return json.load(fh)
-    return None
-
-
-def main(argv=None) -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--timing-steps", type=int, default=20)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--tuning", default="pinned", choices=("pinned", "auto"),
-                    help="pinned: the stage kernel's tile and z-chunk that the autotuner picks on "
-                         "this workload in most runs (FP32-bulk 64x8 tiles, FP64 32x8, z-chunk 64; "
-                         "32x16 is within 2%% and wins some runs), so that the line and a profile of "
-                         "the same command time the same kernel; auto: the autotuner")
-    a = ap.parse_args(argv)
-    if a.gpus != 1:
-        print(f"bench.py: the force-free step runs on one GPU only (asked for --gpus {a.gpus}); "
-              "sharding the box over several GPUs is designed but not built or measured.",
-              file=sys.stderr)
-        return 2
+def cpu_sample(n: int, budget_s: float) -> dict:
+    """Timesteps of the reference's field_solver.F90 (oracle/_ref, FP32 as
+    shipped, amdflang -O3) on the Alfvenic workload at n^3 until about
+    `budget_s` seconds have been timed; step loop only."""
+    import numpy as np
+    from oracle.oracle import RefSolver
     import problems
-    t0 = time.time()
-    prob = problems.alfven(a.n, seed=1, kmax=8)
-    print(f"[bench] initial condition {a.n}^3 built in {time.time() - t0:.1f}s", file=sys.stderr,
-          flush=True)
+    if not RefSolver.available(np.float32):
+        raise FileNotFoundError("oracle/_ref not built")
+    p = problems.alfven(n, seed=1, kmax=8)
+    r = RefSolver(p.n, boundary=p.boundary, cc=p.cc, dtype=np.float32)
+    for f, a in p.fields.items():
+        r.put(f, a.astype(np.float32))
+    r.exchange()
+    t = time.perf_counter()
+    r.run(1)
+    el = time.perf_counter() - t
+    nsteps = 1
+    more = min(59, int(budget_s / max(el, 1e-9)) - 1)
+    if more > 0:
+        t = time.perf_counter()
+        r.run(more)
+        el += time.perf_counter() - t
+        nsteps += more
+    return {"cells": p.cells, "steps": nsteps, "seconds": el}
+
+
+def _cpu_workers(n: int, budget_s: float, cpus: list) -> list:
+    """One bench.py --cpu-worker child per CPU, started together."""
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", str(n), str(budget_s),
+                               str(c)], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True) for c in cpus]
+    out = []
+    for pr in procs:
+        so, se = pr.communicate(timeout=budget_s * 6 + 120)
+        if pr.returncode != 0:
+            raise RuntimeError(f"cpu worker failed ({pr.returncode}): {se.strip()[-300:]}")
+        out.append(json.loads(so.strip().splitlines()[-1]))
+    return out
+
+
+def cpu_baseline(n: int, budget_s: float, ncores: int = 8) -> dict:
+    """The reference's solver on the host: one single-core process alone,
+    then `ncores` single-core processes at once (each pinned to its own
+    CPU), summed.  Reported, never required."""
+    res = {"value": None, "unit": "cell-updates/s", "cores": 1, "kind": "reference"}
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
+        one = _cpu_workers(n, budget_s, cpus[:1])[0]
+        res["value"] = one["cells"] * one["steps"] / one["seconds"]
+        res["sample"] = (f"{one['steps']} timestep(s) of the Alfvenic workload at {n}^3, reference "
+                         f"field_solver.F90 built FP32 as shipped (amdflang -O3), single rank, serial "
+                         f"periodic ghost fill, {one['seconds']:.1f} s on one core")
+        use = cpus[:ncores]
+        many = _cpu_workers(n, budget_s, use)
+        res["aggregate"] = {"value": sum(w["cells"] * w["steps"] / w["seconds"] for w in many),
+                            "unit": "cell-updates/s", "cores": len(use),
+                            "sample": f"{len(use)} concurrent single-core processes, each "
+                                      f"{min(w['steps'] for w in many)}-{max(w['steps'] for w in many)} "
+                                      f"timesteps of the same workload, rates summed"}
+    except Exception as e:  # the baseline is reported, never required
+        res["sample"] = f"unavailable: {e}"
+    return res
+
+
+def load_traffic(workload: str, source_hash: str):
+    """The stored calibrated traffic of this workload's stage kernel, if it
+    was measured on the same kernel sources."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None, f"no profiles/traffic_{workload}.json"
+    with open(path) as fh:
+        tr = json.load(fh)
+    if tr.get("source_hash") != source_hash:
+        return None, f"profiles/traffic_{workload}.json was measured on other kernel sources"
+    return tr, f"profiles/traffic_{workload}.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-kernel calibrated)"
+
+
+def measure_workload(name: str, prob, steps: int, warmup: int, timing_steps: int, pinned: bool) -> dict:
+    """Both paths on one workload, FP32-bulk's energy against FP64's."""
+    tuning = {"fp64": (1, 64), "fp32_bulk": (3, 64)}  # tile index (32x8, 64x8), z-chunk
     res = {}
-    pinned = {"fp64": (1, 64), "fp32_bulk": (3, 64)}  # tile index (32x8, 64x8), z-chunk
     for prec in ("fp64", "fp32_bulk"):
-        res[prec] = run_path(prec, prob, a.steps, a.warmup, a.timing_steps,
-                             pinned[prec] if a.tuning == "pinned" else None)
-        print(f"[bench] {prec}: {res[prec]['value']:.4g} cell-updates/s, "
+        res[prec] = run_path(prec, prob, steps, warmup, timing_steps, tuning[prec] if pinned else None)
+        print(f"[bench] {name} {prec}: {res[prec]['value']:.4g} cell-updates/s, "
               f"{res[prec]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     e64, e32 = res["fp64"]["energy_end"], res["fp32_bulk"]["energy_end"]
-    e_rel = abs(e32 - e64) / abs(e64)
     de64 = e64 - res["fp64"]["energy_start"]
     de32 = e32 - res["fp32_bulk"]["energy_start"]
-    de_rel = abs(de32 - de64) / max(abs(de64), 1e-300)
-    head = res["fp32_bulk"]
-    workload = f"alfven{a.n}"
+    res["energy_rel_err"] = abs(e32 - e64) / abs(e64)
+    res["delta_energy_rel_err"] = abs(de32 - de64) / max(abs(de64), 1e-300)
+    return res
+
+
+def roofline_of(head: dict, workload: str) -> dict:
+    import pytff
     roof = {
         "bound": "hbm",
         "achieved": head["stage_kernel"]["achieved_GBps"],
@@ -188,13 +218,71 @@ def main(argv=None) -> int:
         "avg_launch_ms": head["stage_kernel"]["avg_ms"],
         "alg_bytes_per_launch": head["stage_kernel"]["alg_bytes_per_launch"],
     }
-    tr = load_traffic(workload + "_fp32_bulk")
-    if tr:
-        roof["traffic"] = tr.get("bytes_per_launch")
-    # the reference's loops run the slowest (z) index innermost, about 3 us per
-    # cell-update on one core: a bounded sample at 128^3 (same generator and
-    # per-cell work; its smaller planes favour the CPU's caches)
-    cpu = None if a.no_cpu_baseline else cpu_baseline(min(a.n, 128), 12.0)
+    tr, src = load_traffic(workload + "_fp32_bulk", pytff.kernel_source_hash())
+    if tr and tr.get("tuning") == head["tuning"]:
+        roof["traffic"] = tr["bytes_per_launch"]
+    elif tr:
+        src = f"{src}: measured with tuning {tr.get('tuning')}, not {head['tuning']}"
+    roof["traffic_source"] = src
+    return roof
+
+
+def summary(res: dict, head: dict, workload: str) -> dict:
+    return {"value": head["value"], "ms_per_step": head["ms_per_step"], "roofline": roofline_of(head, workload),
+            "roofline_frac_canonical": head["roofline_frac_canonical"],
+            "fp64": {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
+                     "roofline_frac_canonical": res["fp64"]["roofline_frac_canonical"],
+                     "stage_kernel": res["fp64"]["stage_kernel"]},
+            "fp32_bulk_over_fp64": head["value"] / res["fp64"]["value"],
+            "energy_rel_err_fp32_bulk_vs_fp64": res["energy_rel_err"],
+            "delta_energy_rel_err_fp32_bulk_vs_fp64": res["delta_energy_rel_err"],
+            "divb_max": {"fp64": res["fp64"]["divb_max_end"], "fp32_bulk": head["divb_max_end"]}}
+
+
+def main(argv=None) -> int:
+    argv = sys.argv[1:] if argv is None else argv
+    if argv[:1] == ["--cpu-worker"]:
+        n, budget, cpu = int(argv[1]), float(argv[2]), int(argv[3])
+        os.sched_setaffinity(0, {cpu})
+        print(json.dumps(cpu_sample(n, budget)))
+        return 0
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--timing-steps", type=int, default=20)
+    ap.add_argument("--psr-n", type=int, default=512, help="rotator box (0: leave the rotator out)")
+    ap.add_argument("--psr-steps", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU work per baseline process")
+    ap.add_argument("--tuning", default="pinned", choices=("pinned", "auto"),
+                    help="pinned: the stage kernel's tile and z-chunk that the autotuner picks on "
+                         "the Alfvenic workload in most runs (FP32-bulk 64x8 tiles, FP64 32x8, z-chunk 64), "
+                         "so that the line and a profile of the same command time the same kernel; "
+                         "auto: the autotuner")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py: the force-free step runs on one GPU only (asked for --gpus {a.gpus}); "
+              "sharding the box over several GPUs is designed but not built or measured.",
+              file=sys.stderr)
+        return 2
+    # the host baseline first: its worker processes start before this
+    # process touches the GPU.  The reference's loops run the slowest (z)
+    # index innermost, about 3 us per cell-update on one core: a bounded
+    # sample at 128^3 (same generator and per-cell work; its smaller planes
+    # favour the CPU's caches)
+    cpu = None if a.no_cpu_baseline else cpu_baseline(min(a.n, 128), a.cpu_seconds)
+    if cpu:
+        print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
+    import problems
+    t0 = time.time()
+    prob = problems.alfven(a.n, seed=1, kmax=8, host=False)
+    workload = f"alfven{a.n}"
+    res = measure_workload(workload, prob, a.steps, a.warmup, a.timing_steps, a.tuning == "pinned")
+    print(f"[bench] {workload} done in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    head = res["fp32_bulk"]
+    main_sum = summary(res, head, workload)
     line = {
         "metric": METRIC,
         "value": head["value"],
@@ -211,17 +299,23 @@ def main(argv=None) -> int:
         "config": {"workload": f"{workload}: B0=z + Alfvenic dB/B0=0.05 (k<=8, seed 1), "
                                f"{a.n}^3 periodic, CC=0.45, SSP-RK3 force-free step",
                    "cells": prob.cells, "g": prob.g, "parallelism": "single GPU"},
-        "roofline": roof,
-        "roofline_frac_canonical": head["roofline_frac_canonical"],
-        "fp64": {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
-                 "roofline_frac_canonical": res["fp64"]["roofline_frac_canonical"],
-                 "stage_kernel": res["fp64"]["stage_kernel"]},
-        "fp32_bulk_over_fp64": head["value"] / res["fp64"]["value"],
-        "energy_rel_err_fp32_bulk_vs_fp64": e_rel,
-        "delta_energy_rel_err_fp32_bulk_vs_fp64": de_rel,
-        "divb_max": {"fp64": res["fp64"]["divb_max_end"], "fp32_bulk": head["divb_max_end"]},
-        "cpu_baseline": cpu,
+        "roofline": main_sum["roofline"],
     }
+    line.update({k: v for k, v in main_sum.items() if k not in ("value", "ms_per_step", "roofline")})
+    if a.psr_n:
+        t0 = time.time()
+        psr = problems.psr(a.psr_n, host=False)
+        wl = f"psr{a.psr_n}"
+        rp = measure_workload(wl, psr, a.psr_steps, 10, 5, a.tuning == "pinned")
+        print(f"[bench] {wl} done in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+        ps = summary(rp, rp["fp32_bulk"], wl)
+        ps["steps"] = a.psr_steps
+        ps["config"] = {"workload": f"{wl}: aligned rotator, dipole radius 20 cells, light cylinder 80 cells, "
+                                    f"40-cell fixed absorbing layer, {a.psr_n}^3, CC=0.5",
+                        "cells": psr.cells, "g": psr.g}
+        ps["divb_note"] = "max over every interior cell, including those the conducting-sphere hook writes"
+        line["workloads"] = {wl: ps}
+    line["cpu_baseline"] = cpu
     print(json.dumps(line))
     return 0
 

@@ -186,6 +186,30 @@ int tff_output_samples(tff_handle* h, int istep, void* const out[7], int host_dt
  * handles) race. */
 int tff_output_fields_noalloc(tff_handle* h, int istep, void* const out[6], int host_dtype);
 
+/* Initial conditions set in HBM, as the user files' userInitFields set
+ * them, without host arrays: the state E, B and the background B0 of the
+ * whole box (ghosts included), each value computed in FP64 and stored in
+ * the handle's precision.  The state is then clean, as after tff_upload.
+ *  TFF_IC_LOOP     user_loop.F90:44-93, params xc, yc1, yc2, zc, bnorm:
+ *                  the two-source potential field, E = 0, B0 = B;
+ *  TFF_IC_PSR      user_psr.F90:39-69, params psr_radius, psr_angle [rad],
+ *                  psr_omega, xc, yc, zc: the dipole on the interior
+ *                  cells, zero ghosts, E = 0, B0 = 0;
+ *  TFF_IC_UNIFORM  user_uniform.F90:43-69, params xc, yc, r0, bnorm:
+ *                  bz = bnorm / (1 + r/r0), E = 0, B0 = B. */
+#define TFF_IC_LOOP 1
+#define TFF_IC_PSR 2
+#define TFF_IC_UNIFORM 3
+int tff_init_fields(tff_handle* h, int kind, const double* params, int nparams);
+/* A periodic field given as Fourier modes (no reference counterpart: the
+ * Alfvenic perturbation of the measured workloads): component q of
+ * interior cell (i, j, k) = background[q] + sum over modes of
+ * Re[c_q exp(2 pi i (kx i/nx + ky j/ny + kz k/nz))], q = ex ey ez bx by bz;
+ * ghosts zero; B0 = background[3..5] everywhere.  Each mode is 15 doubles:
+ * kx, ky, kz (integers, |k| (n + 4) < 2^31, else TFF_EINVAL), then (re, im)
+ * of c for the six components. */
+int tff_init_spectrum(tff_handle* h, const double* modes, int nmodes, const double background[6]);
+
 /* FP64 diagnostics over the interior (no reference counterpart):
  * out[0] = 1/2 sum E^2, out[1] = 1/2 sum B^2, out[2] = max |div_h B|,
  * out[3] = sqrt(mean (div_h B)^2). */

@@ -21,6 +21,7 @@
 #include "tff.h"
 #include "tff_batch.h"
 #include "tff_output.h"
+#include "tff_init.h"
 
 using namespace tff;
 
@@ -933,6 +934,72 @@ struct DoReduce {
   }
 };
 
+// After tff_init_fields / tff_init_spectrum wrote the (clean) state and B0
+// in place, as an upload of every field would.
+template <class S>
+int after_init(tff_handle* h, const char* what) {
+  int rc = last_launch(h, what);
+  if (rc) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->ghosts_settled = false;
+  h->t0_dirty = true;
+  return TFF_OK;
+}
+
+template <class PR>
+struct DoInitFields {
+  static int run(tff_handle* h, int kind, const double* params, int nparams) {
+    using S = typename PR::S;
+    double* dp = nullptr;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (hipMalloc(&dp, 16 * sizeof(double)) != hipSuccess) return fail(h, TFF_ENOMEM, "tff_init_fields");
+    double p[16] = {0};
+    for (int q = 0; q < nparams; ++q) p[q] = params[q];
+    int rc = TFF_OK;
+    if (hipMemcpyAsync(dp, p, sizeof p, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+      rc = fail(h, TFF_EHIP, "tff_init_fields: parameters");
+    const int64_t n = (int64_t)(h->g.nx + 2 * NG) * (h->g.ny + 2 * NG) * (h->g.nz + 2 * NG);
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (!rc) {
+      auto e = mv3<S>(h->es), b = mv3<S>(h->bs), b0 = mv3<S>(h->b0);
+      if (kind == TFF_IC_LOOP)
+        hipLaunchKernelGGL(k_init_loop<S>, dim3(nb), dim3(256), 0, h->stream, h->g, e, b, b0, dp, n);
+      else if (kind == TFF_IC_PSR)
+        hipLaunchKernelGGL(k_init_psr<S>, dim3(nb), dim3(256), 0, h->stream, h->g, e, b, b0, dp, n);
+      else
+        hipLaunchKernelGGL(k_init_uniform<S>, dim3(nb), dim3(256), 0, h->stream, h->g, e, b, b0, dp, n);
+      rc = after_init<S>(h, "k_init_fields");
+    }
+    hipFree(dp);
+    return rc;
+  }
+};
+
+template <class PR>
+struct DoInitSpectrum {
+  static int run(tff_handle* h, const double* modes, int nmodes, const double* background) {
+    using S = typename PR::S;
+    double* dm = nullptr;
+    const size_t mb = ((size_t)nmodes * SPEC_MODE + 6) * sizeof(double);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (hipMalloc(&dm, mb) != hipSuccess) return fail(h, TFF_ENOMEM, "tff_init_spectrum");
+    int rc = TFF_OK;
+    if (hipMemcpyAsync(dm, background, 6 * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        (nmodes > 0 && hipMemcpyAsync(dm + 6, modes, (size_t)nmodes * SPEC_MODE * sizeof(double),
+                                      hipMemcpyHostToDevice, h->stream) != hipSuccess))
+      rc = fail(h, TFF_EHIP, "tff_init_spectrum: modes");
+    const int64_t n = (int64_t)(h->g.nx + 2 * NG) * (h->g.ny + 2 * NG) * (h->g.nz + 2 * NG);
+    if (!rc) {
+      hipLaunchKernelGGL(k_init_spectrum<S>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->g,
+                         mv3<S>(h->es), mv3<S>(h->bs), mv3<S>(h->b0), (const double*)(dm + 6), nmodes,
+                         (const double*)dm, n);
+      rc = after_init<S>(h, "k_init_spectrum");
+    }
+    hipFree(dm);
+    return rc;
+  }
+};
+
 template <class PR>
 struct TileLds {
   static int run(tff_handle* h, int t, size_t* out) {
@@ -1407,6 +1474,29 @@ int tff_get_tuning(tff_handle* h, int* tile, int* tx, int* ty, int* kchunk) {
   return TFF_OK;
 }
 
+int tff_init_fields(tff_handle* h, int kind, const double* params, int nparams) {
+  if (!h || (nparams > 0 && !params)) return TFF_EINVAL;
+  static const int need[4] = {0, 5, 6, 4};
+  if (kind < TFF_IC_LOOP || kind > TFF_IC_UNIFORM || nparams < need[kind] || nparams > 16)
+    return fail(h, TFF_EINVAL, "tff_init_fields: unknown kind or wrong number of parameters");
+  int rc = ensure_clean(h);  // the buffers then hold the reference's roles
+  if (rc) return rc;
+  return dispatch<DoInitFields>(h, kind, params, nparams);
+}
+int tff_init_spectrum(tff_handle* h, const double* modes, int nmodes, const double background[6]) {
+  if (!h || nmodes < 0 || (nmodes > 0 && !modes) || !background) return TFF_EINVAL;
+  const double kcap = 2147483647.0 / (std::max(h->g.nx, std::max(h->g.ny, h->g.nz)) + 2 * NG);
+  for (int m = 0; m < nmodes; ++m)
+    for (int a = 0; a < 3; ++a) {
+      const double k = modes[(size_t)m * SPEC_MODE + a];
+      if (!(std::fabs(k) < kcap) || k != std::floor(k))
+        return fail(h, TFF_EINVAL, "tff_init_spectrum: mode " + std::to_string(m) + ": wave number " +
+                                       std::to_string(k) + " is not an integer below " + std::to_string(kcap));
+    }
+  int rc = ensure_clean(h);
+  if (rc) return rc;
+  return dispatch<DoInitSpectrum>(h, modes, nmodes, background);
+}
 int tff_get_size(tff_handle* h, int n[3], int* precision) {
   if (!h || !n) return TFF_EINVAL;
   n[0] = h->g.nx;

@@ -5,7 +5,12 @@ userReadInput + userInitFields; the Fortran host has its own user files).
 Every builder returns a ``Problem``: the field arrays in the reference's
 layout (C-ordered ``(nz+4, ny+4, nx+4)`` views of Fortran arrays with two
 ghost layers, FP64 on the host), the boundary flags, CC, the absorbing-layer
-settings and the boundary-kernel kind and parameters.
+settings and the boundary-kernel kind and parameters.  Every problem except
+the plane wave also carries the recipe of its initial condition for the
+device (``init``): ``load(..., device=True)`` has libtff generate the fields
+in HBM (tff_init_fields, tff_init_spectrum), and a builder called with
+``host=False`` builds no host arrays at all, for boxes whose FP64 host copy
+would take gigabytes.
 """
 from __future__ import annotations
 
@@ -17,6 +22,8 @@ import numpy as np
 NG = 2
 BC_NONE, BC_LOOP, BC_PSR, BC_UNIFORM = 0, 1, 2, 3
 NAMES = ("ex", "ey", "ez", "bx", "by", "bz", "b0x", "b0y", "b0z")
+# the device-side initial conditions (tff_init_fields' kinds)
+IC_LOOP, IC_PSR, IC_UNIFORM = 1, 2, 3
 
 
 @dataclasses.dataclass
@@ -32,6 +39,8 @@ class Problem:
     absorb_fixed: bool = False
     steps: int = 0
     g: int = 0  # 1 when B0 has a non-zero curl (algorithmic bytes count it)
+    # ("fields", kind, params) or ("spectrum", modes, background), or None
+    init: tuple = None
 
     @property
     def cells(self) -> int:
@@ -114,40 +123,71 @@ def emwave_closed_form(p: Problem, amp, mode, nsteps, b0=1.0, axis="z"):
 
 
 # --- config: uniform B0 plus seeded Alfvenic perturbation -----------------
-def _band_limited(n, rng, kmin=1, kmax=8):
-    k = np.fft.fftfreq(n, 1.0 / n)
-    kz, ky, kx = np.meshgrid(k, k, k, indexing="ij")
-    kk = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
-    spec = (rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n)))
-    spec[(kk < kmin) | (kk > kmax)] = 0.0
-    return np.fft.ifftn(spec).real
-
-
-def alfven(n=256, seed=1, amp=0.05, kmax=8, cc=0.45, steps=500) -> Problem:
-    """B = z^ + curl_h A with A band-limited (1 <= |k| <= kmax), so div_h B = 0
-    at t=0; E = grad_perp psi = -(v x z^) for the solenoidal drift
-    v = (d_y psi, -d_x psi, 0).  rms|dB| = rms|E| = amp."""
-    nn = (n, n, n)
+def alfven_modes(n, seed=1, amp=0.05, kmax=8):
+    """The seeded spectrum of alfven(): one row of 15 per wave vector k of
+    the half space (kz > 0, or kz = 0 and ky > 0, or kz = ky = 0 and kx > 0)
+    with 1 <= |k| <= kmax, holding kx ky kz and the complex amplitudes of
+    ex ey ez bx by bz.  Each k draws complex normal amplitudes for a vector
+    potential A and a stream function psi; B's are those of the forward-
+    difference curl of A (so div_h B = 0 to rounding) and E's those of the
+    forward-difference gradient of psi in the x-y plane.  With 2 kmax < n no
+    two modes alias and none is its own conjugate, so the box mean of a
+    mode's square is |c|^2 / 2 (Parseval) and the scales below make
+    rms|dB| = rms|E| = amp exactly."""
+    nx, ny, nz = n
+    if 2 * kmax >= min(n):
+        raise ValueError(f"kmax {kmax} needs a box wider than {2 * kmax} cells")
+    r = np.arange(-kmax, kmax + 1)
+    kz, ky, kx = (a.ravel() for a in np.meshgrid(r, r, r, indexing="ij"))
+    half = (kz > 0) | ((kz == 0) & ((ky > 0) | ((ky == 0) & (kx > 0))))
+    k2 = kx * kx + ky * ky + kz * kz
+    sel = half & (k2 >= 1) & (k2 <= kmax * kmax)
+    kx, ky, kz = kx[sel], ky[sel], kz[sel]
     rng = np.random.default_rng(seed)
-    f = _zeros(nn)
-    ax, ay, az = (_band_limited(n, rng, 1, kmax) for _ in range(3))
-    # arrays are [k][j][i]; roll(-1, axis) gives the +1 neighbour
-    sh = lambda a, ax_: np.roll(a, -1, axis=ax_)
-    dbx = (sh(az, 1) - az) - (sh(ay, 0) - ay)
-    dby = (sh(ax, 0) - ax) - (sh(az, 2) - az)
-    dbz = (sh(ay, 2) - ay) - (sh(ax, 1) - ax)
-    s = amp / math.sqrt(np.mean(dbx ** 2 + dby ** 2 + dbz ** 2))
-    psi = _band_limited(n, rng, 1, kmax)
-    ex = sh(psi, 2) - psi
-    ey = sh(psi, 1) - psi
-    se = amp / math.sqrt(np.mean(ex ** 2 + ey ** 2))
-    _interior(f["bx"])[...] = s * dbx
-    _interior(f["by"])[...] = s * dby
-    _interior(f["bz"])[...] = 1.0 + s * dbz
-    _interior(f["ex"])[...] = se * ex
-    _interior(f["ey"])[...] = se * ey
-    f["b0z"][...] = 1.0
-    return Problem("alfven", nn, (1, 1, 1), cc, f, steps=steps)
+    z = rng.standard_normal((kx.size, 4, 2))
+    z = z[..., 0] + 1j * z[..., 1]
+    a, psi = z[:, :3], z[:, 3]
+    # forward difference of exp(i theta) along each axis
+    dx = np.exp(2j * np.pi * kx / nx) - 1.0
+    dy = np.exp(2j * np.pi * ky / ny) - 1.0
+    dz = np.exp(2j * np.pi * kz / nz) - 1.0
+    cb = np.stack([a[:, 2] * dy - a[:, 1] * dz, a[:, 0] * dz - a[:, 2] * dx, a[:, 1] * dx - a[:, 0] * dy], 1)
+    ce = np.stack([psi * dx, psi * dy, 0.0 * psi], 1)
+    cb *= amp / math.sqrt(0.5 * np.sum(np.abs(cb) ** 2))
+    ce *= amp / math.sqrt(0.5 * np.sum(np.abs(ce) ** 2))
+    modes = np.zeros((kx.size, 15))
+    modes[:, 0], modes[:, 1], modes[:, 2] = kx, ky, kz
+    c = np.concatenate([ce, cb], 1)
+    modes[:, 3::2], modes[:, 4::2] = c.real, c.imag
+    return modes
+
+
+def spectrum_fields(n, modes, background):
+    """The host form of tff_init_spectrum: component q of interior cell
+    (i, j, k) = background[q] + sum over modes of Re[c_q exp(2 pi i (kx i /
+    nx + ky j / ny + kz k / nz))], ghosts zero, B0 = the background B
+    everywhere; the mode sum is one inverse FFT per component."""
+    nx, ny, nz = n
+    f = _zeros(n)
+    idx = tuple(modes[:, a].astype(np.int64) % m for a, m in ((2, nz), (1, ny), (0, nx)))
+    for q, name in enumerate(NAMES[:6]):
+        spec = np.zeros((nz, ny, nx), dtype=np.complex128)
+        spec[idx] = modes[:, 3 + 2 * q] + 1j * modes[:, 4 + 2 * q]
+        _interior(f[name])[...] = background[q] + np.fft.ifftn(spec).real * (nx * ny * nz)
+    for c in range(3):
+        f[NAMES[6 + c]][...] = background[3 + c]
+    return f
+
+
+def alfven(n=256, seed=1, amp=0.05, kmax=8, cc=0.45, steps=500, host=True) -> Problem:
+    """B = z^ + dB, E = the in-plane gradient of a stream function: the
+    seeded band-limited spectrum of alfven_modes (1 <= |k| <= kmax), with
+    div_h B = 0 at t = 0 and rms|dB| = rms|E| = amp; periodic box."""
+    nn = (n, n, n) if np.isscalar(n) else tuple(n)
+    modes = alfven_modes(nn, seed, amp, kmax)
+    bg = (0.0, 0.0, 0.0, 0.0, 0.0, 1.0)
+    f = spectrum_fields(nn, modes, bg) if host else None
+    return Problem("alfven", nn, (1, 1, 1), cc, f, steps=steps, init=("spectrum", modes, bg))
 
 
 # --- config: twisted loop (user_loop.F90 form) -----------------------------
@@ -155,7 +195,7 @@ LOOP_COPY, LOOP_GHOST_DRIVER, LOOP_DRIVEN = 0, 1, 2
 
 
 def loop(n=384, L0=None, a_L0=3.0, h_L0=2.0, r_L0=0.5, cc=0.45, twist_angle=None, seed=3,
-         t0=None, steps=1000, twist=True, driver=LOOP_GHOST_DRIVER, dtype=np.float64) -> Problem:
+         t0=None, steps=1000, twist=True, driver=LOOP_GHOST_DRIVER, dtype=np.float64, host=True) -> Problem:
     """user_loop.F90:27-93: two opposite sources at depth h*L0 below the
     floor, +-a*L0 either side of the box centre; B0 = B everywhere including
     ghosts; x, y periodic, z open with the floor/top hook.  L0 scales as
@@ -183,30 +223,33 @@ def loop(n=384, L0=None, a_L0=3.0, h_L0=2.0, r_L0=0.5, cc=0.45, twist_angle=None
     zc = -h0 * L0
     bnorm = d(1000.0) * h0 ** 2 * L0 ** 2
     r0 = d(r_L0) * L0
-    f = _zeros(nn)
-    i, j, k = _coords(nn, full=True)
-
-    def comp(rx, ry1, ry2, rz, num1, num2):
-        rr1 = np.sqrt(rx * rx + ry1 * ry1 + rz * rz)
-        rr2 = np.sqrt(rx * rx + ry2 * ry2 + rz * rz)
-        return float(bnorm) * (num1 / rr1 ** 3 - num2 / rr2 ** 3)
-
     xcf, yc1f, yc2f, zcf = float(xc), float(yc1), float(yc2), float(zc)
-    rx, ry1, ry2, rz = i - xcf, j + 0.5 - yc1f, j + 0.5 - yc2f, k + 0.5 - zcf
-    f["bx"][...] = comp(rx, ry1, ry2, rz, rx, rx)
-    rx, ry1, ry2, rz = i + 0.5 - xcf, j - yc1f, j - yc2f, k + 0.5 - zcf
-    f["by"][...] = comp(rx, ry1, ry2, rz, ry1, ry2)
-    rx, ry1, ry2, rz = i + 0.5 - xcf, j + 0.5 - yc1f, j + 0.5 - yc2f, k - zcf
-    f["bz"][...] = comp(rx, ry1, ry2, rz, rz, rz)
-    for c in "xyz":
-        f["b0" + c][...] = f["b" + c]
+    f = None
+    if host:
+        f = _zeros(nn)
+        i, j, k = _coords(nn, full=True)
+
+        def comp(rx, ry1, ry2, rz, num1, num2):
+            rr1 = np.sqrt(rx * rx + ry1 * ry1 + rz * rz)
+            rr2 = np.sqrt(rx * rx + ry2 * ry2 + rz * rz)
+            return float(bnorm) * (num1 / rr1 ** 3 - num2 / rr2 ** 3)
+
+        rx, ry1, ry2, rz = i - xcf, j + 0.5 - yc1f, j + 0.5 - yc2f, k + 0.5 - zcf
+        f["bx"][...] = comp(rx, ry1, ry2, rz, rx, rx)
+        rx, ry1, ry2, rz = i + 0.5 - xcf, j - yc1f, j - yc2f, k + 0.5 - zcf
+        f["by"][...] = comp(rx, ry1, ry2, rz, ry1, ry2)
+        rx, ry1, ry2, rz = i + 0.5 - xcf, j + 0.5 - yc1f, j + 0.5 - yc2f, k - zcf
+        f["bz"][...] = comp(rx, ry1, ry2, rz, rz, rz)
+        for c in "xyz":
+            f["b0" + c][...] = f["b" + c]
     # footpoint angular speed is CC * omega (drift v = c E x B / B^2, E = -omega r x B)
     dist = float(h0 * L0) - (0.0 if driver == LOOP_DRIVEN else 1.0)
     ramp = sum(math.tanh(t / t0) for t in range(steps))
     beta = twist_angle * float(r0) * math.exp(dist / float(r0)) / (cc * cc * max(ramp, 1e-30))
     mode = float(driver) if twist else float(LOOP_COPY)
     params = (xcf, yc1f, yc2f, zcf, float(bnorm), float(r0), beta, float(t0), mode)
-    return Problem("loop", nn, (1, 1, 0), cc, f, BC_LOOP, params, steps=steps, g=1)
+    return Problem("loop", nn, (1, 1, 0), cc, f, BC_LOOP, params, steps=steps, g=1,
+                   init=("fields", IC_LOOP, params[:5]))
 
 
 # --- config: aligned rotator (user_psr.F90 form) ---------------------------
@@ -229,7 +272,7 @@ def dipole_field(params, x, y, z, step=0):
 
 
 def psr(n=512, radius=20.0, r_lc=80.0, angle_deg=0.0, absorb_buff=40, cc=0.5, periods=2.0,
-        absorb_fixed=True, dtype=np.float64) -> Problem:
+        absorb_fixed=True, dtype=np.float64, host=True) -> Problem:
     """user_psr.F90:26-69: dipole of radius R at the box centre (interior
     cells only, ghosts zero, B0 = 0), all boundaries absorbing.  The period
     puts the light cylinder at r_lc cells: P = 2 pi r_lc / c."""
@@ -241,40 +284,56 @@ def psr(n=512, radius=20.0, r_lc=80.0, angle_deg=0.0, absorb_buff=40, cc=0.5, pe
     om = d(2.0) * pi / d(period)
     c = d(0.5) * d(n)
     params = (float(d(radius)), float(ang), float(om), float(c), float(c), float(c))
-    f = _zeros(nn)
-    i, j, k = _coords(nn)
-    bx, _, _ = dipole_field(params, i, j + 0.5, k + 0.5)
-    _, by, _ = dipole_field(params, i + 0.5, j, k + 0.5)
-    _, _, bz = dipole_field(params, i + 0.5, j + 0.5, k)
-    _interior(f["bx"])[...] = bx
-    _interior(f["by"])[...] = by
-    _interior(f["bz"])[...] = bz
+    f = None
+    if host:
+        f = _zeros(nn)
+        i, j, k = _coords(nn)
+        bx, _, _ = dipole_field(params, i, j + 0.5, k + 0.5)
+        _, by, _ = dipole_field(params, i + 0.5, j, k + 0.5)
+        _, _, bz = dipole_field(params, i + 0.5, j + 0.5, k)
+        _interior(f["bx"])[...] = bx
+        _interior(f["by"])[...] = by
+        _interior(f["bz"])[...] = bz
     steps = int(round(periods * period))
     return Problem("psr", nn, (2, 2, 2), cc, f, BC_PSR, params, absorb_buff=absorb_buff,
-                   absorb_fixed=absorb_fixed, steps=steps)
+                   absorb_fixed=absorb_fixed, steps=steps, init=("fields", IC_PSR, params))
 
 
 # --- user_uniform.F90 form ---------------------------------------------------
-def uniform(n=64, L0=10.0, r_L0=1.0, beta_twist=0.1, t0=50.0, cc=0.45, steps=100) -> Problem:
+def uniform(n=64, L0=10.0, r_L0=1.0, beta_twist=0.1, t0=50.0, cc=0.45, steps=100, host=True) -> Problem:
     """user_uniform.F90:27-69: bz = 1/(1 + r/r0) about the vertical axis
     through the box centre, B0 = B, x, y periodic and z open; the hook
     rotates the floor (z <= 0)."""
     nn = (n, n, n) if np.isscalar(n) else tuple(n)
     xc, yc = 0.5 * nn[0], 0.5 * nn[1]
     r0 = r_L0 * L0
-    f = _zeros(nn)
-    i, j, k = _coords(nn, full=True)
-    rr = np.sqrt((i + 0.5 - xc) ** 2 + (j + 0.5 - yc) ** 2 + 1e-7)
-    f["bz"][...] = 1.0 / (1.0 + rr / r0) + 0 * k
-    f["b0z"][...] = f["bz"]
+    f = None
+    if host:
+        f = _zeros(nn)
+        i, j, k = _coords(nn, full=True)
+        rr = np.sqrt((i + 0.5 - xc) ** 2 + (j + 0.5 - yc) ** 2 + 1e-7)
+        f["bz"][...] = 1.0 / (1.0 + rr / r0) + 0 * k
+        f["b0z"][...] = f["bz"]
     params = (xc, yc, r0, beta_twist, t0, 1.0)
-    return Problem("uniform", nn, (1, 1, 0), cc, f, BC_UNIFORM, params, steps=steps, g=1)
+    return Problem("uniform", nn, (1, 1, 0), cc, f, BC_UNIFORM, params, steps=steps, g=1,
+                   init=("fields", IC_UNIFORM, (xc, yc, r0, 1.0)))
 
 
-def load(target, p: Problem):
+def load(target, p: Problem, device=False):
     """Upload a problem into a Meshblock or an oracle (both expose set_fields
-    and set_bc)."""
-    target.set_fields(**p.fields)
+    and set_bc).  device=True (or a problem built with host=False) has the
+    Meshblock generate the initial fields in HBM from p.init instead."""
+    if device or p.fields is None:
+        if p.init is None:
+            raise ValueError(f"problem {p.name} has no device initial condition")
+        if not hasattr(target, "init_fields"):
+            raise TypeError(f"{type(target).__name__} cannot generate fields; build the problem with host=True")
+        if p.init[0] == "spectrum":
+            target.init_spectrum(p.init[1], p.init[2])
+        else:
+            target.init_fields(p.init[1], p.init[2])
+    else:
+        target.set_fields(**p.fields)
     target.set_bc(p.bc_kind, p.bc_params)
 
 

@@ -56,6 +56,20 @@ class Instance(ctypes.Structure):
 _lib = None
 
 
+def kernel_source_hash() -> str:
+    """A short SHA-256 over the HIP sources libtff.so is built from, so that
+    a stored counter profile can be matched to the kernels it measured."""
+    import hashlib
+    d = os.path.join(PKG, "csrc")
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(d)):
+        if name.endswith((".h", ".hip")):
+            h.update(name.encode())
+            with open(os.path.join(d, name), "rb") as fh:
+                h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def header_functions() -> list[str]:
     """Names of the functions include/tff.h declares."""
     txt = open(HEADER).read()
@@ -90,6 +104,12 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.tff_output_dims.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.tff_output_fields.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
     lib.tff_output_fields_noalloc.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
+    lib.tff_output_samples.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
+    lib.tff_init_fields.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    lib.tff_init_spectrum.argtypes = [P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    lib.tff_set_phase_timing.argtypes = [P, ctypes.c_int]
+    lib.tff_phase_times.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
+    lib.tff_get_size.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.tff_sync.argtypes = [P]
     lib.tff_event_record.argtypes = [P, ctypes.c_int]
     lib.tff_event_elapsed.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
@@ -253,6 +273,36 @@ class Meshblock:
         fn = "tff_output_fields_noalloc" if noalloc else "tff_output_fields"
         self._check(getattr(self.lib, fn)(self.h, int(istep), ptrs, dt), fn)
         return dict(zip(("ex", "ey", "ez", "bx", "by", "bz"), arrs))
+
+    def output_samples(self, istep: int, dtype=np.float32) -> dict:
+        """Raw ex..bz and rho at the output nodes (tff_output_samples)."""
+        d = (ctypes.c_int * 3)()
+        self._check(self.lib.tff_output_dims(self.h, int(istep), d), "tff_output_dims")
+        arrs = [np.empty((d[2], d[1], d[0]), dtype=dtype) for _ in range(7)]
+        ptrs = (ctypes.c_void_p * 7)(*[a.ctypes.data for a in arrs])
+        dt = 1 if np.dtype(dtype) == np.float32 else 0
+        self._check(self.lib.tff_output_samples(self.h, int(istep), ptrs, dt), "tff_output_samples")
+        return dict(zip(("ex", "ey", "ez", "bx", "by", "bz", "rho"), arrs))
+
+    def init_fields(self, kind: int, params):
+        """tff_init_fields: a user file's initial condition set in HBM."""
+        p = (ctypes.c_double * 16)(*[float(x) for x in params])
+        self._check(self.lib.tff_init_fields(self.h, int(kind), p, len(params)), "tff_init_fields")
+
+    def init_spectrum(self, modes: np.ndarray, background):
+        """tff_init_spectrum: a periodic field from Fourier modes (rows of 15)."""
+        m = np.ascontiguousarray(modes, dtype=np.float64).reshape(-1, 15)
+        bg = (ctypes.c_double * 6)(*[float(x) for x in background])
+        self._check(self.lib.tff_init_spectrum(self.h, m.ctypes.data, m.shape[0], bg), "tff_init_spectrum")
+
+    def phase_timing(self, on: bool):
+        self._check(self.lib.tff_set_phase_timing(self.h, int(on)), "tff_set_phase_timing")
+
+    def phase_times(self) -> list[float]:
+        """GPU ms of the solver, hook and exchange phases since the last call."""
+        ms = (ctypes.c_double * 3)()
+        self._check(self.lib.tff_phase_times(self.h, ms), "tff_phase_times")
+        return list(ms)
 
     def sync(self):
         self._check(self.lib.tff_sync(self.h), "tff_sync")
