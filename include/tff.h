@@ -46,6 +46,16 @@ extern "C" {
                            U^n + c ((X - U^n) + dU), exact in scale; cells
                            near a magnetic null redo their projection and
                            currents in FP64; FP64 reductions */
+#define TFF_FP32_COMP 3 /* FP32-bulk stencils on a compensated state: each
+                           field value is a pair of FP32 words hi + lo, the
+                           stencils read hi, the stage accumulation adds the
+                           increment to the pair by two-sum and the E.B
+                           projection works on hi + lo in FP64; twice the
+                           storage; the per-procedure kernels only (no fused
+                           path, no batches).  An FP64 upload splits each
+                           value into the pair, an FP64 download returns
+                           hi + lo.  Cells a boundary hook writes keep the
+                           hook's FP32 value with the lo they had. */
 
 /* boundary flags, src/objects/domain.F90:31-33 and initialize.F90:139-148 */
 #define TFF_OPEN 0
@@ -93,7 +103,7 @@ typedef struct tff_config {
                           floor (:300); 1: fixed -- jg from j, delta clamped
                           to 1 (fields beyond r_max are zeroed) and cleanEpar
                           skips the projection where |B|^2 is exactly 0 */
-  int precision;       /* TFF_FP64, TFF_FP32, TFF_FP32_BULK */
+  int precision;       /* TFF_FP64, TFF_FP32, TFF_FP32_BULK, TFF_FP32_COMP */
   int device;          /* HIP device ordinal */
   double cc;           /* <algorithm> c, the CFL number CC (global_namespace.F90:14) */
 } tff_config;

@@ -97,6 +97,9 @@ struct tff_handle {
   // changes (see ghosts_static): the fused step then runs only the hook
   bool ghosts_settled = false;
   int fused = 1;
+  // TFF_FP32_COMP: every field buffer carries lo words after its hi words
+  // (alloc_vec doubles it); the unfused path only
+  bool comp = false;
   int tile = -1;
   int kchunk = 0;
   int max_lds = 65536;
@@ -156,7 +159,7 @@ int fail(tff_handle* h, int code, const std::string& msg) {
 // stream (hipMemset, hipMemcpy) would not be ordered before its kernels.
 int alloc_vec(tff_handle* h, Vec& v) {
   void* p = nullptr;
-  const size_t nb = 3 * (size_t)h->nalloc * h->esz;
+  const size_t nb = (h->comp ? 6 : 3) * (size_t)h->nalloc * h->esz;
   if (hipMalloc(&p, nb) != hipSuccess) return fail(h, TFF_ENOMEM, "hipMalloc of a field buffer failed");
   if (hipMemsetAsync(p, 0, nb, h->stream) != hipSuccess) {
     hipFree(p);
@@ -237,9 +240,19 @@ int launch_rk3(tff_handle* h, const Vec& xe, const Vec& xb, const Vec& ue, const
     e1 = h->tpool[h->tused++];
     HIPCHK(h, hipEventRecord(e0, h->stream));
   }
-  hipLaunchKernelGGL(k_rk3<PR>, box_grid(h), dim3(BX, BY), 0, h->stream, h->g, cv3<S>(xe), cv3<S>(xb),
-                     cv3<S>(ue), cv3<S>(ub), cv3<S>(h->t0), h->has_t0, mv3<S>(ye), mv3<S>(yb), cc,
-                     k1, k2, k3, ab);
+  if constexpr (std::is_same<PR, PrecF32Bulk>::value && sizeof(P) == 4) {
+    if (h->comp) {
+      Absorb<float> af{ab.on, ab.fixed, (float)ab.xc, (float)ab.yc, (float)ab.zc, (float)ab.rabsorb,
+                       (float)ab.buff};
+      hipLaunchKernelGGL(k_rk3c<PR>, box_grid(h), dim3(BX, BY), 0, h->stream, h->g, cv3<S>(xe), cv3<S>(xb),
+                         cv3<S>(ue), cv3<S>(ub), cv3<S>(h->t0), h->has_t0, mv3<S>(ye), mv3<S>(yb), (float)cc,
+                         (float)k2, (float)k3, af, 3 * h->g.cs);
+    }
+  }
+  if (!h->comp)
+    hipLaunchKernelGGL(k_rk3<PR>, box_grid(h), dim3(BX, BY), 0, h->stream, h->g, cv3<S>(xe), cv3<S>(xb),
+                       cv3<S>(ue), cv3<S>(ub), cv3<S>(h->t0), h->has_t0, mv3<S>(ye), mv3<S>(yb), cc,
+                       k1, k2, k3, ab);
   int rc = last_launch(h, "k_rk3");
   if (rc) return rc;
   if (h->timing) HIPCHK(h, hipEventRecord(e1, h->stream));
@@ -250,6 +263,13 @@ template <class PR>
 int launch_clean(tff_handle* h, const Vec& xe, const Vec& xb, const Vec& ye) {
   using S = typename PR::S;
   PhaseScope phase(h, TFF_PHASE_SOLVER);
+  if constexpr (std::is_same<S, float>::value) {
+    if (h->comp) {
+      hipLaunchKernelGGL(k_cleanc, box_grid(h), dim3(BX, BY), 0, h->stream, h->g, cv3<S>(xe), cv3<S>(xb),
+                         mv3<S>(ye), h->cfg.absorb_fixed, 3 * h->g.cs);
+      return last_launch(h, "k_cleanc");
+    }
+  }
   hipLaunchKernelGGL(k_clean<PR>, box_grid(h), dim3(BX, BY), 0, h->stream, h->g, cv3<S>(xe),
                      cv3<S>(xb), mv3<S>(ye), h->cfg.absorb_fixed);
   return last_launch(h, "k_clean");
@@ -260,18 +280,25 @@ int launch_fill(tff_handle* h, std::initializer_list<const Vec*> vecs) {
   const int* b = h->cfg.boundary;
   if (b[0] != TFF_PERIODIC && b[1] != TFF_PERIODIC && b[2] != TFF_PERIODIC) return TFF_OK;
   PhaseScope phase(h, TFF_PHASE_EXCHANGE);
-  FillSet<S> fs{};
-  fs.n = 0;
+  std::vector<S*> arr;
   for (const Vec* v : vecs)
-    for (int c = 0; c < 3; ++c) fs.a[fs.n++] = (S*)v->p[c];
+    for (int c = 0; c < (h->comp ? 6 : 3); ++c) arr.push_back((S*)v->p[0] + (int64_t)c * h->g.cs);
   const int sx = h->g.nx + 2 * NG, sy = h->g.ny + 2 * NG;
   const int64_t nshell = (int64_t)2 * NG * sx * sy + (int64_t)2 * NG * sx * h->g.nz +
                          (int64_t)2 * NG * h->g.ny * h->g.nz;
   const int tb = 256;
-  hipLaunchKernelGGL(k_fill<S>, dim3((unsigned)((nshell + tb - 1) / tb)), dim3(tb), 0, h->stream,
-                     h->g, b[0] == TFF_PERIODIC, b[1] == TFF_PERIODIC, b[2] == TFF_PERIODIC, fs,
-                     nshell);
-  return last_launch(h, "k_fill");
+  constexpr int CAP = (int)(sizeof(FillSet<S>::a) / sizeof(S*));
+  for (size_t a0 = 0; a0 < arr.size(); a0 += CAP) {
+    FillSet<S> fs{};
+    fs.n = 0;
+    for (size_t q = a0; q < arr.size() && fs.n < CAP; ++q) fs.a[fs.n++] = arr[q];
+    hipLaunchKernelGGL(k_fill<S>, dim3((unsigned)((nshell + tb - 1) / tb)), dim3(tb), 0, h->stream,
+                       h->g, b[0] == TFF_PERIODIC, b[1] == TFF_PERIODIC, b[2] == TFF_PERIODIC, fs,
+                       nshell);
+    int rc = last_launch(h, "k_fill");
+    if (rc) return rc;
+  }
+  return TFF_OK;
 }
 
 template <class PR>
@@ -283,17 +310,18 @@ int launch_bc(tff_handle* h, const Vec& e, const Vec& b, const int* step_ptr, in
   const int sx = h->g.nx + 2 * NG, sy = h->g.ny + 2 * NG;
   const int tb = 256;
   const P cc = std::is_same<S, P>::value ? (P)(S)h->cfg.cc : (P)h->cfg.cc;
+  const int64_t lo = h->comp ? 3 * h->g.cs : 0;
   switch (h->bc_kind) {
     case TFF_BC_LOOP: {
       const int64_t n = (int64_t)(2 * NG + 1) * sx * sy;
       hipLaunchKernelGGL(k_bc_loop<PR>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, h->stream,
-                         h->g, mv3<S>(e), mv3<S>(b), h->d_bc, cc, step_ptr, step_val);
+                         h->g, mv3<S>(e), mv3<S>(b), h->d_bc, cc, step_ptr, step_val, lo);
       return last_launch(h, "k_bc_loop");
     }
     case TFF_BC_UNIFORM: {
       const int64_t n = (int64_t)(2 * NG + 1) * sx * sy;
       hipLaunchKernelGGL(k_bc_uniform<PR>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0,
-                         h->stream, h->g, mv3<S>(e), mv3<S>(b), h->d_bc, cc, step_ptr, step_val);
+                         h->stream, h->g, mv3<S>(e), mv3<S>(b), h->d_bc, cc, step_ptr, step_val, lo);
       return last_launch(h, "k_bc_uniform");
     }
     case TFF_BC_PSR: {
@@ -307,7 +335,7 @@ int launch_bc(tff_handle* h, const Vec& e, const Vec& b, const int* step_ptr, in
       if (rc) return rc;
       hipLaunchKernelGGL(k_bc_psr_commit<PR>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0,
                          h->stream, h->g, mv3<S>(e), mv3<S>(b), h->d_bc, h->psr_lo, h->psr_n,
-                         (const S*)h->psr_stage);
+                         (const S*)h->psr_stage, lo);
       return last_launch(h, "k_bc_psr_commit");
     }
     default:
@@ -922,7 +950,7 @@ struct DoReduce {
   static int run(tff_handle* h, double* out) {
     using S = typename PR::S;
     hipLaunchKernelGGL(k_reduce_partial<S>, dim3(h->red_blocks), dim3(RED_THREADS), 0, h->stream,
-                       h->g, cv3<S>(h->es), cv3<S>(h->bs), h->d_part);
+                       h->g, cv3<S>(h->es), cv3<S>(h->bs), h->d_part, h->comp ? 3 * h->g.cs : (int64_t)0);
     int rc = last_launch(h, "k_reduce_partial");
     if (rc) return rc;
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, h->stream, h->d_part, h->red_blocks,
@@ -962,12 +990,13 @@ struct DoInitFields {
     const unsigned nb = (unsigned)((n + 255) / 256);
     if (!rc) {
       auto e = mv3<S>(h->es), b = mv3<S>(h->bs), b0 = mv3<S>(h->b0);
+      const int64_t lo = h->comp ? 3 * h->g.cs : 0;
       if (kind == TFF_IC_LOOP)
-        hipLaunchKernelGGL(k_init_loop<S>, dim3(nb), dim3(256), 0, h->stream, h->g, e, b, b0, dp, n);
+        hipLaunchKernelGGL(k_init_loop<S>, dim3(nb), dim3(256), 0, h->stream, h->g, e, b, b0, dp, n, lo);
       else if (kind == TFF_IC_PSR)
-        hipLaunchKernelGGL(k_init_psr<S>, dim3(nb), dim3(256), 0, h->stream, h->g, e, b, b0, dp, n);
+        hipLaunchKernelGGL(k_init_psr<S>, dim3(nb), dim3(256), 0, h->stream, h->g, e, b, b0, dp, n, lo);
       else
-        hipLaunchKernelGGL(k_init_uniform<S>, dim3(nb), dim3(256), 0, h->stream, h->g, e, b, b0, dp, n);
+        hipLaunchKernelGGL(k_init_uniform<S>, dim3(nb), dim3(256), 0, h->stream, h->g, e, b, b0, dp, n, lo);
       rc = after_init<S>(h, "k_init_fields");
     }
     hipFree(dp);
@@ -992,7 +1021,7 @@ struct DoInitSpectrum {
     if (!rc) {
       hipLaunchKernelGGL(k_init_spectrum<S>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->g,
                          mv3<S>(h->es), mv3<S>(h->bs), mv3<S>(h->b0), (const double*)(dm + 6), nmodes,
-                         (const double*)dm, n);
+                         (const double*)dm, n, h->comp ? 3 * h->g.cs : (int64_t)0);
       rc = after_init<S>(h, "k_init_spectrum");
     }
     hipFree(dm);
@@ -1058,7 +1087,7 @@ int tff_create(const tff_config* cfg, tff_handle** out) {
   if (!cfg || !out) return TFF_EINVAL;
   *out = nullptr;
   if (cfg->nx < NG || cfg->ny < NG || cfg->nz < NG) return TFF_EINVAL;  // initialize.F90:256-264
-  if (cfg->precision < TFF_FP64 || cfg->precision > TFF_FP32_BULK) return TFF_EINVAL;
+  if (cfg->precision < TFF_FP64 || cfg->precision > TFF_FP32_COMP) return TFF_EINVAL;
   for (int a = 0; a < 3; ++a)
     if (cfg->boundary[a] < 0 || cfg->boundary[a] > 2) return TFF_EINVAL;
   if (!(cfg->cc > 0.0) || cfg->absorb_buff < 0) return TFF_EINVAL;
@@ -1068,6 +1097,8 @@ int tff_create(const tff_config* cfg, tff_handle** out) {
   if (cfg->boundary[0] == TFF_ABSORB || cfg->boundary[1] == TFF_ABSORB || cfg->boundary[2] == TFF_ABSORB)
     h->cfg.boundary[0] = h->cfg.boundary[1] = h->cfg.boundary[2] = TFF_ABSORB;
   h->esz = cfg->precision == TFF_FP64 ? 8 : 4;
+  h->comp = cfg->precision == TFF_FP32_COMP;
+  if (h->comp) h->fused = 0;
   const int al = (int)(128 / h->esz);  // elements per 128-byte line
   h->xo = al;
   h->g.nx = cfg->nx;
@@ -1145,6 +1176,39 @@ void tff_destroy(tff_handle* h) {
 
 const char* tff_last_error(const tff_handle* h) { return h ? h->err.c_str() : "null handle"; }
 
+// Transfers of a compensated handle: an FP64 upload splits each value into
+// hi = fl32(x) and lo = fl32(x - hi), an FP32 upload sets lo = 0; an FP64
+// download returns hi + lo (exact in FP64), an FP32 download hi.
+static int xfer_pairs(tff_handle* h, void* buf, int comp, void* host, bool f64, bool up, int sx, int sy, int sz) {
+  const size_t n = (size_t)sx * sy * sz;
+  std::vector<float> hi(n), lo(n);
+  char* dhi = (char*)buf + ((size_t)comp * h->g.cs + (size_t)(h->xo - NG)) * 4;
+  char* dlo = dhi + (size_t)3 * h->g.cs * 4;
+  const size_t pitch = (size_t)h->g.px * 4, row = (size_t)sx * 4;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (up) {
+    for (size_t t = 0; t < n; ++t) {
+      const double x = f64 ? ((const double*)host)[t] : (double)((const float*)host)[t];
+      hi[t] = (float)x;
+      lo[t] = (float)(x - (double)hi[t]);
+    }
+    HIPCHK(h, hipMemcpy2DAsync(dhi, pitch, hi.data(), row, row, (size_t)sy * sz, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpy2DAsync(dlo, pitch, lo.data(), row, row, (size_t)sy * sz, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return TFF_OK;
+  }
+  HIPCHK(h, hipMemcpy2DAsync(hi.data(), row, dhi, pitch, row, (size_t)sy * sz, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpy2DAsync(lo.data(), row, dlo, pitch, row, (size_t)sy * sz, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (size_t t = 0; t < n; ++t) {
+    if (f64)
+      ((double*)host)[t] = (double)hi[t] + (double)lo[t];
+    else
+      ((float*)host)[t] = hi[t];
+  }
+  return TFF_OK;
+}
+
 static int xfer(tff_handle* h, int field, void* host, int host_dtype, bool up) {
   if (!h || !host) return TFF_EINVAL;
   if (field < TFF_EX || field > TFF_BNZ) return fail(h, TFF_EINVAL, "unknown field id");
@@ -1163,6 +1227,7 @@ static int xfer(tff_handle* h, int field, void* host, int host_dtype, bool up) {
   const int sx = h->g.nx + 2 * NG, sy = h->g.ny + 2 * NG, sz = h->g.nz + 2 * NG;
   const size_t n = (size_t)sx * sy * sz;
   const size_t hsz = host_dtype == TFF_HOST_F64 ? 8 : 4;
+  if (h->comp) return xfer_pairs(h, v->p[0], comp, host, hsz == 8, up, sx, sy, sz);
   std::vector<unsigned char> tmp;
   void* hp = host;
   if (hsz != h->esz) {
@@ -1265,10 +1330,15 @@ int tff_init_rk_step(tff_handle* h) {
     hipLaunchKernelGGL(k_copy3<double>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, h->stream, n,
                        cv3<double>(h->bs), mv3<double>(h->bn));
   } else {
-    hipLaunchKernelGGL(k_copy3<float>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, h->stream, n,
-                       cv3<float>(h->es), mv3<float>(h->en));
-    hipLaunchKernelGGL(k_copy3<float>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, h->stream, n,
-                       cv3<float>(h->bs), mv3<float>(h->bn));
+    // (the compensated pairs: the lo words follow as three more components)
+    for (int64_t lo = 0; lo <= (h->comp ? 3 * h->g.cs : 0); lo += 3 * h->g.cs) {
+      auto sh = [lo](V3<float> v) { return V3<float>{v.x + lo, v.y + lo, v.z + lo}; };
+      auto shc = [lo](V3<const float> v) { return V3<const float>{v.x + lo, v.y + lo, v.z + lo}; };
+      hipLaunchKernelGGL(k_copy3<float>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, h->stream, n,
+                         shc(cv3<float>(h->es)), sh(mv3<float>(h->en)));
+      hipLaunchKernelGGL(k_copy3<float>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, h->stream, n,
+                         shc(cv3<float>(h->bs)), sh(mv3<float>(h->bn)));
+    }
   }
   return last_launch(h, "k_copy3");
 }
@@ -1442,6 +1512,7 @@ int tff_set_kernel_timing(tff_handle* h, int on) {
 
 int tff_set_path(tff_handle* h, int fused) {
   if (!h) return TFF_EINVAL;
+  if (h->comp && fused) return fail(h, TFF_EINVAL, "TFF_FP32_COMP runs the per-procedure kernels only");
   int rc = ensure_clean(h);
   if (rc) return rc;
   h->fused = fused != 0;
@@ -1971,6 +2042,7 @@ extern "C" {
 int tff_batch_create(const tff_instance* in, int n, int precision, int device, tff_batch** out) {
   if (!in || n < 1 || !out) return TFF_EINVAL;
   *out = nullptr;
+  if (precision == TFF_FP32_COMP) return TFF_EINVAL;  // the batched kernels have no lo words
   tff_batch* b = new tff_batch();
   b->precision = precision;
   b->device = device;

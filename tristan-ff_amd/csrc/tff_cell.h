@@ -426,7 +426,29 @@ __device__ __forceinline__ void update_cell(typename PR::P* v, const typename PR
       v[m] = PR::null_fp64 ? mad_<true>(c3, d[m], mad_<true>(c2, v[m], c1 * u[m]))
                            : c1 * u[m] + c2 * v[m] + c3 * d[m];
   }
-  if (ab.on) {
+  constexpr bool fast_layer = PR::null_fp64 && sizeof(P) == 4;
+  if (fast_layer && ab.on) {
+    // FP32-bulk: the same layer from six squared distances that share
+    // their per-axis squares, hardware square roots (1 ulp) and a multiply
+    // by 1/buff (S = P here, so there is no storage rounding to apply)
+    const P ig = (P)i, jg = ab.fixed ? (P)j : (P)i, kg = (P)k;
+    const P x0 = ig - ab.xc, y0 = jg - ab.yc, z0 = kg - ab.zc;
+    const P x1 = x0 + (P)0.5, y1 = y0 + (P)0.5, z1 = z0 + (P)0.5;
+    const P xx0 = x0 * x0, yy0 = y0 * y0, zz0 = z0 * z0, xx1 = x1 * x1, yy1 = y1 * y1, zz1 = z1 * z1;
+    const P lim = ab.rabsorb - (P)1.0;
+    if (lim < (P)0.0 || xx0 + yy0 + zz0 > lim * lim) {
+      const P ib = __builtin_amdgcn_rcpf(ab.buff);
+      // Yee offsets of ex ey ez bx by bz
+      const P r2[6] = {xx1 + yy0 + zz0, xx0 + yy1 + zz0, xx0 + yy0 + zz1,
+                       xx0 + yy1 + zz1, xx1 + yy0 + zz1, xx1 + yy1 + zz0};
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        P dd = (__builtin_amdgcn_sqrtf(r2[m]) - ab.rabsorb) * ib;
+        if (ab.fixed) dd = dd < (P)1.0 ? dd : (P)1.0;
+        if (dd > (P)0.0) v[m] = mad_<true>(-dd * dd, dd * v[m], v[m]);
+      }
+    }
+  } else if (!fast_layer && ab.on) {
     // the reference stores first and damps the stored value
     for (int m = 0; m < 6; ++m) v[m] = (P)(S)v[m];
     const P ig = (P)i, jg = ab.fixed ? (P)j : (P)i, kg = (P)k;
@@ -447,6 +469,57 @@ __device__ __forceinline__ void update_cell(typename PR::P* v, const typename PR
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Compensated state (TFF_FP32_COMP): every field value is an unevaluated
+// FP32 pair hi + lo with |lo| <= ulp(hi) / 2; stencils read hi, the stage
+// accumulation and the projection carry lo.  The pair arithmetic rests on
+// Knuth's two-sum, an error-free transformation in round-to-nearest (the
+// library is built with -ffp-contract=off, so the compiler keeps its four
+// roundings apart).
+
+// s + e = a + b exactly, s = fl(a + b)
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  const float bv = s - a;
+  e = (a - (s - bv)) + (b - bv);
+}
+
+// The SSP-RK3 stage Y = U + c2 (X - U) + c3 dU (c1 = 1 - c2) on pairs:
+// X_hi - U_hi exactly by two-sum, the increment in FP32 (its own rounding
+// is relative to the increment, not to U), and one two-sum onto U_hi.
+__device__ __forceinline__ void comp_update(float uh, float ul, float xh, float xl, float d, float c2, float c3,
+                                            float& yh, float& yl) {
+  float dh, de;
+  two_sum(xh, -uh, dh, de);
+  const float dl = de + (xl - ul);
+  const float inc = __builtin_fmaf(c2, dh, __builtin_fmaf(c2, dl, c3 * d));
+  two_sum(uh, ul + inc, yh, yl);
+}
+
+// (yh + yl) * f on pairs: the product's rounding error by fma (two-product)
+__device__ __forceinline__ void comp_scale(float& yh, float& yl, float f) {
+  const float p = yh * f;
+  const float e = __builtin_fmaf(yh, f, -p);
+  two_sum(p, __builtin_fmaf(yl, f, e), yh, yl);
+}
+
+// Damping factor of the absorbing layer for component m (Yee offsets of
+// ex ey ez bx by bz) at cell (i, j, k), 1 outside the layer; FP32 with
+// hardware square roots, as update_cell's FP32-bulk layer.
+__device__ __forceinline__ float absorb_factor(const Absorb<float>& ab, int i, int j, int k, int m) {
+  const float hx[6] = {0.5f, 0.f, 0.f, 0.f, 0.5f, 0.5f};
+  const float hy[6] = {0.f, 0.5f, 0.f, 0.5f, 0.f, 0.5f};
+  const float hz[6] = {0.f, 0.f, 0.5f, 0.5f, 0.5f, 0.f};
+  const float ig = (float)i, jg = ab.fixed ? (float)j : (float)i, kg = (float)k;
+  const float x0 = ig - ab.xc, y0 = jg - ab.yc, z0 = kg - ab.zc;
+  const float lim = ab.rabsorb - 1.0f;
+  if (!(lim < 0.0f || x0 * x0 + y0 * y0 + z0 * z0 > lim * lim)) return 1.0f;
+  const float x = x0 + hx[m], y = y0 + hy[m], z = z0 + hz[m];
+  float dd = (__builtin_amdgcn_sqrtf(x * x + y * y + z * z) - ab.rabsorb) * __builtin_amdgcn_rcpf(ab.buff);
+  if (ab.fixed) dd = dd < 1.0f ? dd : 1.0f;
+  return dd > 0.0f ? 1.0f - dd * dd * dd : 1.0f;
 }
 
 }  // namespace tff

@@ -19,25 +19,26 @@ __device__ __forceinline__ bool interior_cell(const Geom& g, int i, int j, int k
   return i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
 }
 
+// Stores the nine values of cell c; lo != 0 (a compensated handle) also
+// stores each value's remainder x - fl(x) `lo` elements after it.
 template <class S>
 __device__ __forceinline__ void store6(const Geom& g, int64_t c, V3<S> e, V3<S> b, V3<S> b0, double ex, double ey,
                                        double ez, double bx, double by, double bz, double b0x, double b0y,
-                                       double b0z) {
-  e.x[c] = (S)ex;
-  e.y[c] = (S)ey;
-  e.z[c] = (S)ez;
-  b.x[c] = (S)bx;
-  b.y[c] = (S)by;
-  b.z[c] = (S)bz;
-  b0.x[c] = (S)b0x;
-  b0.y[c] = (S)b0y;
-  b0.z[c] = (S)b0z;
+                                       double b0z, int64_t lo) {
+  S* dst[9] = {e.x, e.y, e.z, b.x, b.y, b.z, b0.x, b0.y, b0.z};
+  const double val[9] = {ex, ey, ez, bx, by, bz, b0x, b0y, b0z};
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    const S h = (S)val[q];
+    dst[q][c] = h;
+    if (lo) dst[q][c + lo] = (S)(val[q] - (double)h);
+  }
 }
 
 // user_loop.F90:44-93: two opposite sources at (xc, yc1|yc2, zc); every cell
 // including the ghosts; E = 0, B0 = B.  p: xc, yc1, yc2, zc, bnorm.
 template <class S>
-__global__ void k_init_loop(Geom g, V3<S> e, V3<S> b, V3<S> b0, const double* p, int64_t ncell) {
+__global__ void k_init_loop(Geom g, V3<S> e, V3<S> b, V3<S> b0, const double* p, int64_t ncell, int64_t lo) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ncell) return;
   int i, j, k;
@@ -55,14 +56,14 @@ __global__ void k_init_loop(Geom g, V3<S> e, V3<S> b, V3<S> b0, const double* p,
   const double by = field(rx, ry1, ry2, rz, ry1, ry2);
   rx = x + 0.5 - xc; ry1 = y + 0.5 - yc1; ry2 = y + 0.5 - yc2; rz = z - zc;
   const double bz = field(rx, ry1, ry2, rz, rz, rz);
-  store6(g, g.at(i, j, k), e, b, b0, 0.0, 0.0, 0.0, bx, by, bz, bx, by, bz);
+  store6(g, g.at(i, j, k), e, b, b0, 0.0, 0.0, 0.0, bx, by, bz, bx, by, bz, lo);
 }
 
 // user_psr.F90:39-69: the dipole of radius R at (xc, yc, zc) on the
 // interior cells at phase 0, zero in the ghosts; E = 0, B0 = 0.
 // p: R, angle [rad], omega (unused), xc, yc, zc.
 template <class S>
-__global__ void k_init_psr(Geom g, V3<S> e, V3<S> b, V3<S> b0, const double* p, int64_t ncell) {
+__global__ void k_init_psr(Geom g, V3<S> e, V3<S> b, V3<S> b0, const double* p, int64_t ncell, int64_t lo) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ncell) return;
   int i, j, k;
@@ -74,14 +75,15 @@ __global__ void k_init_psr(Geom g, V3<S> e, V3<S> b, V3<S> b0, const double* p, 
     dipole<double>(p, 0, i + 0.5, (double)j, k + 0.5, ox, by, oz);
     dipole<double>(p, 0, i + 0.5, j + 0.5, (double)k, ox, oy, bz);
   }
-  store6(g, g.at(i, j, k), e, b, b0, 0.0, 0.0, 0.0, bx, by, bz, 0.0, 0.0, 0.0);
+  store6(g, g.at(i, j, k), e, b, b0, 0.0, 0.0, 0.0, bx, by, bz, 0.0, 0.0, 0.0, lo);
 }
 
 // user_uniform.F90:43-69: bz = bnorm / (1 + r/r0) about the vertical axis
 // through (xc, yc), every cell including the ghosts; E = 0, B0 = B.
 // p: xc, yc, r0, bnorm.
 template <class S>
-__global__ void k_init_uniform(Geom g, V3<S> e, V3<S> b, V3<S> b0, const double* p, int64_t ncell) {
+__global__ void k_init_uniform(Geom g, V3<S> e, V3<S> b, V3<S> b0, const double* p, int64_t ncell,
+                               int64_t lo) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ncell) return;
   int i, j, k;
@@ -89,7 +91,7 @@ __global__ void k_init_uniform(Geom g, V3<S> e, V3<S> b, V3<S> b0, const double*
   const double rx = i + 0.5 - p[0], ry = j + 0.5 - p[1];
   const double rr = sqrt(rx * rx + ry * ry + 1e-7);  // TINY, src/defs.F90:4
   const double bz = p[3] / (1.0 + rr / p[2]);
-  store6(g, g.at(i, j, k), e, b, b0, 0.0, 0.0, 0.0, 0.0, 0.0, bz, 0.0, 0.0, bz);
+  store6(g, g.at(i, j, k), e, b, b0, 0.0, 0.0, 0.0, 0.0, 0.0, bz, 0.0, 0.0, bz, lo);
 }
 
 // A sum of Fourier modes on the periodic box: component q of cell (i, j, k)
@@ -101,7 +103,7 @@ __global__ void k_init_uniform(Geom g, V3<S> e, V3<S> b, V3<S> b0, const double*
 constexpr int SPEC_MODE = 15;
 template <class S>
 __global__ void k_init_spectrum(Geom g, V3<S> e, V3<S> b, V3<S> b0, const double* modes, int nmodes,
-                                const double* bg, int64_t ncell) {
+                                const double* bg, int64_t ncell, int64_t lo) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ncell) return;
   int i, j, k;
@@ -127,7 +129,7 @@ __global__ void k_init_spectrum(Geom g, V3<S> e, V3<S> b, V3<S> b0, const double
   } else {
     for (int q = 0; q < 6; ++q) v[q] = 0.0;
   }
-  store6(g, g.at(i, j, k), e, b, b0, v[0], v[1], v[2], v[3], v[4], v[5], bg[3], bg[4], bg[5]);
+  store6(g, g.at(i, j, k), e, b, b0, v[0], v[1], v[2], v[3], v[4], v[5], bg[3], bg[4], bg[5], lo);
 }
 
 }  // namespace tff

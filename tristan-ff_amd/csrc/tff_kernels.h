@@ -113,6 +113,99 @@ __global__ void __launch_bounds__(BX* BY)
 }
 
 // ---------------------------------------------------------------------------
+// The compensated path (TFF_FP32_COMP): each field buffer holds its lo
+// words `lo` elements after the hi words (hi x y z, then lo x y z).
+//
+// rk3Step on pairs: the push evaluates the FP32-bulk right-hand side from
+// the hi words (with its FP64 redo near magnetic nulls); the update and the
+// absorbing layer carry lo (comp_update, comp_scale).  c1 = 1 - c2, as for
+// every SSP-RK3 stage.
+template <class PR>
+__global__ void __launch_bounds__(BX* BY)
+    k_rk3c(Geom g, V3<const float> xe, V3<const float> xb, V3<const float> ue, V3<const float> ub,
+           V3<const float> t0, int has_t0, V3<float> ye, V3<float> yb, float cc, float c2, float c3,
+           Absorb<float> ab, int64_t lo) {
+  const int i = blockIdx.x * BX + threadIdx.x - NG;
+  const int j = blockIdx.y * BY + threadIdx.y - NG;
+  const int k = blockIdx.z - NG;
+  if (i > g.nx - 1 + NG || j > g.ny - 1 + NG) return;
+  const int64_t c = g.at(i, j, k);
+  const float* xs[6] = {xe.x, xe.y, xe.z, xb.x, xb.y, xb.z};
+  float* ys[6] = {ye.x, ye.y, ye.z, yb.x, yb.y, yb.z};
+  if (!interior(g, i, j, k)) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      ys[m][c] = xs[m][c];
+      ys[m][c + lo] = xs[m][c + lo];
+    }
+    return;
+  }
+  const int64_t sy = g.px, sz = g.plane;
+  auto off = [=](int a, int b, int d) { return c + a + b * sy + d * sz; };
+  auto EX = [&](int a, int b, int d) { return xe.x[off(a, b, d)]; };
+  auto EY = [&](int a, int b, int d) { return xe.y[off(a, b, d)]; };
+  auto EZ = [&](int a, int b, int d) { return xe.z[off(a, b, d)]; };
+  auto BXF = [&](int a, int b, int d) { return xb.x[off(a, b, d)]; };
+  auto BYF = [&](int a, int b, int d) { return xb.y[off(a, b, d)]; };
+  auto BZF = [&](int a, int b, int d) { return xb.z[off(a, b, d)]; };
+  auto RHO = [&](int a, int b, int d) { return rho_node<float>(EX, EY, EZ, a, b, d); };
+  const float tx = has_t0 ? t0.x[c] : 0.f, ty = has_t0 ? t0.y[c] : 0.f, tz = has_t0 ? t0.z[c] : 0.f;
+  typename PR::P d[6];
+  rhs_cell<PR>(EX, EY, EZ, BXF, BYF, BZF, RHO, cc, tx, ty, tz, d);
+  const float* us[6] = {ue.x, ue.y, ue.z, ub.x, ub.y, ub.z};
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    float yh, yl;
+    comp_update(us[m][c], us[m][c + lo], xs[m][c], xs[m][c + lo], (float)d[m], c2, c3, yh, yl);
+    if (ab.on) comp_scale(yh, yl, absorb_factor(ab, i, j, k, m));
+    ys[m][c] = yh;
+    ys[m][c + lo] = yl;
+  }
+}
+
+// cleanEpar on pairs: the projection in FP64 from hi + lo of every E and B
+// word it reads (so the parallel part of lo is removed too), the result
+// split back into a pair.  `guard` as in k_clean.
+__global__ void __launch_bounds__(BX* BY)
+    k_cleanc(Geom g, V3<const float> xe, V3<const float> xb, V3<float> ye, int guard, int64_t lo) {
+  const int i = blockIdx.x * BX + threadIdx.x - NG;
+  const int j = blockIdx.y * BY + threadIdx.y - NG;
+  const int k = blockIdx.z - NG;
+  if (i > g.nx - 1 + NG || j > g.ny - 1 + NG) return;
+  const int64_t c = g.at(i, j, k);
+  const float* xs[3] = {xe.x, xe.y, xe.z};
+  float* ys[3] = {ye.x, ye.y, ye.z};
+  if (!interior(g, i, j, k)) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      ys[m][c] = xs[m][c];
+      ys[m][c + lo] = xs[m][c + lo];
+    }
+    return;
+  }
+  const int64_t sy = g.px, sz = g.plane;
+  auto pair = [=](const float* a, int x, int y, int z) {
+    const int64_t o = c + x + y * sy + z * sz;
+    return (double)a[o] + (double)a[o + lo];
+  };
+  auto EX = [&](int a, int b, int d) { return pair(xe.x, a, b, d); };
+  auto EY = [&](int a, int b, int d) { return pair(xe.y, a, b, d); };
+  auto EZ = [&](int a, int b, int d) { return pair(xe.z, a, b, d); };
+  auto BXF = [&](int a, int b, int d) { return pair(xb.x, a, b, d); };
+  auto BYF = [&](int a, int b, int d) { return pair(xb.y, a, b, d); };
+  auto BZF = [&](int a, int b, int d) { return pair(xb.z, a, b, d); };
+  const double o[3] = {clean_point_fp64<0>(EX, EY, EZ, BXF, BYF, BZF, guard),
+                       clean_point_fp64<1>(EX, EY, EZ, BXF, BYF, BZF, guard),
+                       clean_point_fp64<2>(EX, EY, EZ, BXF, BYF, BZF, guard)};
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const float h = (float)o[m];
+    ys[m][c] = h;
+    ys[m][c + lo] = (float)(o[m] - (double)h);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Ghost shell enumeration: z-slabs (all i, j), then y-slabs (interior k, all
 // i), then x-slabs (interior j, k).  Every ghost cell appears exactly once.
 // A shell has far fewer than 2^32 cells, so the index arithmetic is 32-bit
@@ -159,7 +252,7 @@ __device__ __forceinline__ bool shell_cell(const Geom& g, int64_t t64, int& i, i
 // and it takes the value of the wrapped interior cell.
 template <class S>
 struct FillSet {
-  S* a[9];
+  S* a[12];
   int n;
 };
 
@@ -249,9 +342,18 @@ __device__ __forceinline__ P loop_top_bz(const double* p, int i, int j, int k) {
 
 template <class PR>
 __global__ void k_bc_loop(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, const double* p,
-                          typename PR::P cc, const int* step_ptr, int step_val) {
+                          typename PR::P cc, const int* step_ptr, int step_val, int64_t lo) {
   using S = typename PR::S;
   using P = typename PR::P;
+  // lo != 0 (compensated pairs): a written value's lo is 0, a copy's is the source's
+  auto put = [lo](S* a, int64_t c, P v) {
+    a[c] = (S)v;
+    if (lo) a[c + lo] = (S)0;
+  };
+  auto cpy = [lo](S* a, int64_t c, int64_t src) {
+    a[c] = a[src];
+    if (lo) a[c + lo] = a[src + lo];
+  };
   const int sx = g.nx + 2 * NG, sy = g.ny + 2 * NG;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)(2 * NG + 1) * sx * sy) return;
@@ -267,24 +369,24 @@ __global__ void k_bc_loop(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, co
     if (mode != 0) {
       P ex, ey, bz;
       loop_driver<P>(p, cc, step_ptr ? *step_ptr + step_val : step_val, i, j, k, ex, ey, bz);
-      e.x[c] = (S)ex;
-      e.y[c] = (S)ey;
-      b.z[c] = (S)bz;
+      put(e.x, c, ex);
+      put(e.y, c, ey);
+      put(b.z, c, bz);
     } else {
       const int64_t s = g.at(i, j, 0);
-      e.x[c] = e.x[s];
-      e.y[c] = e.y[s];
-      b.z[c] = b.z[s];
+      cpy(e.x, c, s);
+      cpy(e.y, c, s);
+      cpy(b.z, c, s);
     }
   } else if (mode == 2) {
-    b.z[c] = (S)loop_top_bz<P>(p, i, j, k);
-    e.x[c] = (S)0;
-    e.y[c] = (S)0;
+    put(b.z, c, loop_top_bz<P>(p, i, j, k));
+    put(e.x, c, (P)0);
+    put(e.y, c, (P)0);
   } else {
     const int64_t s = g.at(i, j, g.nz - 1);
-    e.x[c] = e.x[s];
-    e.y[c] = e.y[s];
-    b.z[c] = b.z[s];
+    cpy(e.x, c, s);
+    cpy(e.y, c, s);
+    cpy(b.z, c, s);
   }
 }
 
@@ -293,9 +395,13 @@ __global__ void k_bc_loop(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, co
 // included, as z_glob .le. 0 does) and the top ghost planes.
 template <class PR>
 __global__ void k_bc_uniform(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, const double* p,
-                             typename PR::P cc, const int* step_ptr, int step_val) {
+                             typename PR::P cc, const int* step_ptr, int step_val, int64_t lo) {
   using S = typename PR::S;
   using P = typename PR::P;
+  auto put = [lo](S* a, int64_t c, P v) {
+    a[c] = (S)v;
+    if (lo) a[c + lo] = (S)0;
+  };
   const int sx = g.nx + 2 * NG, sy = g.ny + 2 * NG;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)(2 * NG + 1) * sx * sy) return;
@@ -314,21 +420,21 @@ __global__ void k_bc_uniform(Geom g, V3<typename PR::S> e, V3<typename PR::S> b,
     rr = sqrt(rx * rx + ry * ry + tiny);
     omega = exp(-rr / r0) * beta * cc / r0;
     ee0 = -omega * bnorm * tanh(time / t0);
-    e.x[c] = (S)(rx * ee0);
+    put(e.x, c, rx * ee0);
     rx = xg - xc; ry = yg + hf - yc;
     rr = sqrt(rx * rx + ry * ry + tiny);
     omega = exp(-rr / r0) * beta * cc / r0;
     ee0 = -omega * bnorm * tanh(time / t0);
-    e.y[c] = (S)(ry * ee0);
+    put(e.y, c, ry * ee0);
     rx = xg + hf - xc; ry = yg + hf - yc;
     rr = sqrt(rx * rx + ry * ry + tiny);
-    b.z[c] = (S)(bnorm / (one + (rr / r0)));
+    put(b.z, c, bnorm / (one + (rr / r0)));
   } else {
     rx = xg + hf - xc; ry = yg + hf - yc;
     rr = sqrt(rx * rx + ry * ry + tiny);
-    b.z[c] = (S)(bnorm / (one + (rr / r0)));
-    e.x[c] = (S)0;
-    e.y[c] = (S)0;
+    put(b.z, c, bnorm / (one + (rr / r0)));
+    put(e.x, c, (P)0);
+    put(e.y, c, (P)0);
   }
 }
 
@@ -487,7 +593,7 @@ __global__ void k_bc_psr_eval(Geom g, V3<const typename PR::S> e, V3<const typen
 
 template <class PR>
 __global__ void k_bc_psr_commit(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, const double* p,
-                                int3 lo, int3 n, const typename PR::S* nw) {
+                                int3 lo, int3 n, const typename PR::S* nw, int64_t pair) {
   using P = typename PR::P;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nbox = (int64_t)n.x * n.y * n.z;
@@ -501,12 +607,12 @@ __global__ void k_bc_psr_commit(Geom g, V3<typename PR::S> e, V3<typename PR::S>
   const P fi = (P)i, fj = (P)j, fk = (P)k;
   if (!((fi - xc) * (fi - xc) + (fj - yc) * (fj - yc) + (fk - zc) * (fk - zc) < ssq)) return;
   const int64_t c = g.at(i, j, k);
-  e.x[c] = nw[0 * nbox + t];
-  e.y[c] = nw[1 * nbox + t];
-  e.z[c] = nw[2 * nbox + t];
-  b.x[c] = nw[3 * nbox + t];
-  b.y[c] = nw[4 * nbox + t];
-  b.z[c] = nw[5 * nbox + t];
+  typename PR::S* dst[6] = {e.x, e.y, e.z, b.x, b.y, b.z};
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    dst[m][c] = nw[m * nbox + t];
+    if (pair) dst[m][c + pair] = (typename PR::S)0;  // compensated pairs: the blend's lo is 0
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -527,7 +633,7 @@ __device__ __forceinline__ void wave_reduce(double& a, double& b, double& c, dou
 
 template <class S>
 __global__ void __launch_bounds__(RED_THREADS)
-    k_reduce_partial(Geom g, V3<const S> e, V3<const S> b, double* part) {
+    k_reduce_partial(Geom g, V3<const S> e, V3<const S> b, double* part, int64_t lo) {
   const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
   double se = 0.0, sb = 0.0, sd = 0.0, md = 0.0;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell;
@@ -536,12 +642,13 @@ __global__ void __launch_bounds__(RED_THREADS)
     const int j = (int)((t / g.nx) % g.ny);
     const int k = (int)(t / ((int64_t)g.nx * g.ny));
     const int64_t c = g.at(i, j, k);
-    const double ax = e.x[c], ay = e.y[c], az = e.z[c];
-    const double bx = b.x[c], by = b.y[c], bz = b.z[c];
+    // lo != 0: the compensated pairs, hi + lo (exact in FP64)
+    auto v = [&](const S* a, int64_t o) { return lo ? (double)a[o] + (double)a[o + lo] : (double)a[o]; };
+    const double ax = v(e.x, c), ay = v(e.y, c), az = v(e.z, c);
+    const double bx = v(b.x, c), by = v(b.y, c), bz = v(b.z, c);
     se += ax * ax + ay * ay + az * az;
     sb += bx * bx + by * by + bz * bz;
-    const double dv = ((double)b.x[c + 1] - bx) + ((double)b.y[c + g.px] - by) +
-                      ((double)b.z[c + g.plane] - bz);
+    const double dv = (v(b.x, c + 1) - bx) + (v(b.y, c + g.px) - by) + (v(b.z, c + g.plane) - bz);
     sd += dv * dv;
     md = fmax(md, fabs(dv));
   }

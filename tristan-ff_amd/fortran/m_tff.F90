@@ -4,7 +4,7 @@ module m_tff
   implicit none
 
   integer(c_int), parameter :: TFF_OK = 0
-  integer(c_int), parameter :: TFF_FP64 = 0, TFF_FP32 = 1, TFF_FP32_BULK = 2
+  integer(c_int), parameter :: TFF_FP64 = 0, TFF_FP32 = 1, TFF_FP32_BULK = 2, TFF_FP32_COMP = 3
   integer(c_int), parameter :: TFF_OPEN = 0, TFF_PERIODIC = 1, TFF_ABSORB = 2
   integer(c_int), parameter :: TFF_EX = 0, TFF_EY = 1, TFF_EZ = 2, TFF_BX = 3, TFF_BY = 4, &
                                TFF_BZ = 5, TFF_B0X = 6, TFF_B0Y = 7, TFF_B0Z = 8

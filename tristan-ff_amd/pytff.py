@@ -26,13 +26,13 @@ LIB_PATH = os.environ.get("TFF_LIBRARY") or os.path.join(PKG, "_build", "libtff.
 HEADER = os.path.join(ROOT, "include", "tff.h")
 NG = 2
 
-FP64, FP32, FP32_BULK = 0, 1, 2
+FP64, FP32, FP32_BULK, FP32_COMP = 0, 1, 2, 3
 OPEN, PERIODIC, ABSORB = 0, 1, 2
 BC_NONE, BC_LOOP, BC_PSR, BC_UNIFORM = 0, 1, 2, 3
 FIELDS = {"ex": 0, "ey": 1, "ez": 2, "bx": 3, "by": 4, "bz": 5,
           "b0x": 6, "b0y": 7, "b0z": 8, "enx": 9, "eny": 10, "enz": 11,
           "bnx": 12, "bny": 13, "bnz": 14}
-_PREC_NAMES = {"fp64": FP64, "fp32": FP32, "fp32_bulk": FP32_BULK}
+_PREC_NAMES = {"fp64": FP64, "fp32": FP32, "fp32_bulk": FP32_BULK, "fp32_comp": FP32_COMP}
 
 
 class TffError(RuntimeError):
