@@ -127,8 +127,9 @@ int tff_download(tff_handle* h, int field, void* host, int host_dtype);
  *                  (user_loop.F90:27-42 and :101-164).  mode 0: the
  *                  floor/top copy as shipped; 1: the footpoint driver of the
  *                  commented block :122-150 on the floor ghosts; 2: that
- *                  driver on z <= 0 (floor plane included) with a conducting
- *                  top, the boundary form of user_uniform.F90:113-142
+ *                  driver's E on z <= 0 (floor plane included, its bz left
+ *                  to the update so div B is kept) with a conducting top,
+ *                  the boundary form of user_uniform.F90:113-142
  *  TFF_BC_PSR      psr_radius, psr_angle [rad], psr_omega, xc, yc, zc
  *                  (user_psr.F90:26-37 and :72-332)
  *  TFF_BC_UNIFORM  xc, yc, r0, beta_twist, t0, bnorm

@@ -390,8 +390,13 @@ static void orc_bc_loop(orc_t *o, int step) {
   const real beta = (real)o->bcp[6], t0 = (real)o->bcp[7];
   /* twist 0: floor/top copy as shipped; 1: the commented footpoint driver on
    * the floor ghosts (z < 0); 2: builder-defined driven loop in the form of
-   * user_uniform.F90:113-142 -- the driver on z <= 0 (the floor plane
-   * included) and a conducting top (ex = ey = 0, bz = the potential field) */
+   * user_uniform.F90:113-142 -- the driver's ex, ey on z <= 0 (the floor
+   * plane included) and a conducting top (ex = ey = 0, bz = the potential
+   * field).  Unlike user_uniform.F90:134-137, bz of the floor plane itself
+   * is not pinned: it follows Faraday's law under the driver's E, so the
+   * floor cells keep div_h B (pinning it while the rotation's E has a
+   * non-zero discrete curl changes div_h B there every stage); bz is
+   * pinned on the ghost planes z < 0 only. */
   const int twist = (int)o->bcp[8];
   const real time = (real)step;
   const real CC = o->cc;
@@ -422,7 +427,7 @@ static void orc_bc_loop(orc_t *o, int step) {
             rx = xg + (real)0.5 - xc; ry1 = yg + (real)0.5 - yc1; ry2 = yg + (real)0.5 - yc2; rz = zg - zc;
             rr1 = (real)sqrt(rx * rx + ry1 * ry1 + rz * rz);
             rr2 = (real)sqrt(rx * rx + ry2 * ry2 + rz * rz);
-            bz[c] = bnorm * (rz / cube(rr1) - rz / cube(rr2));
+            if (zg < (real)0) bz[c] = bnorm * (rz / cube(rr1) - rz / cube(rr2));
           } else {
             ex[c] = ex[IDX(o, i, j, 0)];
             ey[c] = ey[IDX(o, i, j, 0)];

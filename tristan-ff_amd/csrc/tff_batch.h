@@ -135,7 +135,7 @@ __global__ void k_batch_ghost_build(const BatchStage<typename PR::S, typename PR
       loop_driver<P>(in.bcp, b.a.cc, *step_ptr, i, j, k, ex, ey, bz);
       e3[0] = (S)ex;
       e3[1] = (S)ey;
-      b3[2] = (S)bz;
+      if (k < 0) b3[2] = (S)bz;
     } else if (mode == 2) {
       // conducting top
       e3[0] = (S)0;
@@ -165,8 +165,9 @@ __global__ void k_batch_ghost_build(const BatchStage<typename PR::S, typename PR
 }
 
 // The interior floor plane of every driven-loop instance (mode 2): the
-// footpoint driver's ex, ey, bz at k = 0, written before the ghost pass so
-// that periodic ghost columns copy the driven values.
+// footpoint driver's ex, ey at k = 0 (bz there follows the update), written
+// before the ghost pass so that periodic ghost columns copy the driven
+// values.
 template <class PR>
 __global__ void k_batch_floor(const BatchStage<typename PR::S, typename PR::P>* st, const BatchInfo* info,
                               const int64_t* floor_base, const int* floor_inst, int nfloor, int64_t total,
@@ -187,7 +188,6 @@ __global__ void k_batch_floor(const BatchStage<typename PR::S, typename PR::P>* 
   const int64_t c = g.at(i, j, 0);
   b.a.oe[c] = (S)ex;
   b.a.oe[c + g.cs] = (S)ey;
-  b.a.ob[c + 2 * g.cs] = (S)bz;
 }
 
 // Cleaned E at every ghost cell of the stage output (k_ghost_clean for the

@@ -299,8 +299,12 @@ __global__ void __launch_bounds__(BX* BY)
 //   1  the footpoint-rotation driver of the commented block :122-150, with
 //      its tanh(time/t0) ramp on, on the floor ghosts; top copy;
 //   2  a driven loop in the boundary form of user_uniform.F90:113-142: the
-//      same driver on z <= 0 (the interior floor plane included) and a
-//      conducting top (ex = ey = 0, bz = the potential field).
+//      driver's ex, ey on z <= 0 (the interior floor plane included), bz of
+//      the potential field on the floor ghosts z < 0, and a conducting top
+//      (ex = ey = 0, bz = the potential field).  bz of the floor plane is
+//      left to Faraday's law under the driver's E (user_uniform.F90 pins it
+//      too): the rotation's E has a non-zero discrete curl, so a pinned bz
+//      would change div_h B of the floor cells every stage.
 // One thread per (i, j) of the ghost-inclusive plane and per plane of
 // k = -2, -1, 0, nz, nz+1; plane 0 is written in mode 2 only.
 template <class P>
@@ -371,7 +375,7 @@ __global__ void k_bc_loop(Geom g, V3<typename PR::S> e, V3<typename PR::S> b, co
       loop_driver<P>(p, cc, step_ptr ? *step_ptr + step_val : step_val, i, j, k, ex, ey, bz);
       put(e.x, c, ex);
       put(e.y, c, ey);
-      put(b.z, c, bz);
+      if (k < 0) put(b.z, c, bz);
     } else {
       const int64_t s = g.at(i, j, 0);
       cpy(e.x, c, s);

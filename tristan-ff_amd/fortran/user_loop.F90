@@ -3,7 +3,8 @@
 ! hook runs on the device (TFF_BC_LOOP); <problem> twist = 1 adds the
 ! footpoint-1 rotation that the reference leaves commented out at
 ! user_loop.F90:122-150 (floor ghosts, r0 = r_L0 * L0, tanh(t/t0) ramp);
-! twist = 2 applies it on z <= 0 with a conducting top instead.
+! twist = 2 applies its E on z <= 0 with a conducting top instead (the
+! floor plane's bz then follows the update, which keeps div B there).
 #define NGHOST 2
 module m_userfile
   use m_globalnamespace

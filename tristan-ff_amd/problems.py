@@ -205,11 +205,12 @@ def loop(n=384, L0=None, a_L0=3.0, h_L0=2.0, r_L0=0.5, cc=0.45, twist_angle=None
     beta_twist is set so that the centre of the footpoint turns by
     twist_angle (seeded in [pi/2, 3pi/2]) over `steps` steps.
 
-    driver = LOOP_DRIVEN instead applies that driver on z <= 0 (the floor
-    plane included) and closes the top as a conductor (ex = ey = 0, bz = the
-    potential field), the boundary form of user_uniform.F90:113-142: the
-    ghost-only driver and the floor/top copy never reach the interior
-    update, so only this form twists the loop."""
+    driver = LOOP_DRIVEN instead applies that driver's ex, ey on z <= 0 (the
+    floor plane included; bz only on the ghosts below it, so the floor's bz
+    follows Faraday's law and div_h B is kept) and closes the top as a
+    conductor (ex = ey = 0, bz = the potential field), the boundary form of
+    user_uniform.F90:113-142: the ghost-only driver and the floor/top copy
+    never reach the interior update, so only this form twists the loop."""
     nn = (n, n, n)
     d = np.dtype(dtype).type
     L0 = 20.0 * n / 256 if L0 is None else L0
@@ -317,6 +318,70 @@ def uniform(n=64, L0=10.0, r_L0=1.0, beta_twist=0.1, t0=50.0, cc=0.45, steps=100
     params = (xc, yc, r0, beta_twist, t0, 1.0)
     return Problem("uniform", nn, (1, 1, 0), cc, f, BC_UNIFORM, params, steps=steps, g=1,
                    init=("fields", IC_UNIFORM, (xc, yc, r0, 1.0)))
+
+
+# --- diagnostics: div_h B -------------------------------------------------------
+def div_h_b(bx, by, bz) -> np.ndarray:
+    """div_h B at the centre of every interior cell, from ghost-inclusive
+    (k, j, i) arrays: (bx(i+1) - bx(i)) + (by(j+1) - by(j)) + (bz(k+1) - bz(k)),
+    the cell-centred divergence of the Yee face fields, in FP64 (the stored
+    ghosts supply the upper faces of the last interior cells)."""
+    s, s1 = slice(NG, -NG), slice(NG + 1, -NG + 1 if NG > 1 else None)
+    d = np.asarray(bx[s, s, s1], dtype=np.float64) - bx[s, s, s]
+    d += np.asarray(by[s, s1, s], dtype=np.float64) - by[s, s, s]
+    d += np.asarray(bz[s1, s, s], dtype=np.float64) - bz[s, s, s]
+    return d
+
+
+def divb_free_mask(p: Problem):
+    """Interior cells whose div_h B only the curl update changes (None: all
+    of them).  Excluded are the cells whose faces a boundary hook writes or
+    the absorbing layer damps component by component: the rotator's
+    conducting-sphere region (centre within 2R + 3 of the star) and every
+    cell whose centre lies within 2 cells of the absorbing layer's inner
+    radius (field_solver.F90:198-269: min(n)/2 - absorb_buff); the floor
+    plane of the user_uniform hook, which pins bz there."""
+    nx, ny, nz = p.n
+    mask = None
+    if p.bc_kind == BC_PSR or 2 in p.boundary:
+        i, j, k = _coords(p.n)
+        mask = np.ones((nz, ny, nx), dtype=bool)
+        if 2 in p.boundary:
+            xc, yc, zc = 0.5 * nx, 0.5 * ny, 0.5 * nz
+            rabs = min(xc, yc, zc) - p.absorb_buff
+            r2 = (i + 0.5 - xc) ** 2 + (j + 0.5 - yc) ** 2 + (k + 0.5 - zc) ** 2
+            mask &= r2 < (rabs - 2.0) ** 2
+            del r2
+        if p.bc_kind == BC_PSR:
+            R, xc, yc, zc = p.bc_params[0], p.bc_params[3], p.bc_params[4], p.bc_params[5]
+            r2 = (i + 0.5 - xc) ** 2 + (j + 0.5 - yc) ** 2 + (k + 0.5 - zc) ** 2
+            mask &= r2 >= (2.0 * R + 3.0) ** 2
+    elif p.bc_kind == BC_UNIFORM:
+        mask = np.ones((nz, ny, nx), dtype=bool)
+        mask[0] = False
+    return mask
+
+
+def divb_report(p: Problem, d0: np.ndarray, d1: np.ndarray, bmax: float) -> dict:
+    """div_h B at t = 0 (d0) and after the run (d1), and its change, over
+    every interior cell and over the cells no hook or absorber writes
+    (divb_free_mask); `bmax` = max |B| at t = 0, the scale the change is
+    quoted against."""
+    m = divb_free_mask(p)
+    dd = np.abs(d1 - d0)
+    out = {"max_t0": float(np.abs(d0).max()), "max_end": float(np.abs(d1).max()),
+           "l2_end": float(np.sqrt(np.mean(d1 * d1))), "change_max": float(dd.max()),
+           "b_max_t0": float(bmax)}
+    if m is not None:
+        out["change_max_free"] = float(dd[m].max()) if m.any() else 0.0
+        out["change_l2_free"] = float(np.sqrt(np.mean(dd[m] ** 2))) if m.any() else 0.0
+        out["free_cells"] = int(m.sum())
+    else:
+        out["change_max_free"] = out["change_max"]
+        out["change_l2_free"] = float(np.sqrt(np.mean(dd * dd)))
+        out["free_cells"] = int(dd.size)
+    out["change_max_free_rel_bmax"] = out["change_max_free"] / bmax if bmax > 0 else 0.0
+    return out
 
 
 def load(target, p: Problem, device=False):

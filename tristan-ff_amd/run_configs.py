@@ -22,8 +22,11 @@ reference's own precision) and for FP64 started from B perturbed by one
 part in 1e12 (how much the workload itself amplifies roundoff), against
 FP64: the energy history's relative
 error (on E_EM and on E_EM - E_EM(t=0)), the normwise max and L2 relative
-error of every component after the last step, and max |div B|.  One JSON
-object per workload is printed, and appended to --out if given.
+error of every component after the last step, and div_h B of each run at
+t = 0 and after the last step with its change over the run (over every
+interior cell and over the cells no hook or absorbing layer writes, for
+which the scheme conserves it).  One JSON object per workload is printed,
+and appended to --out if given.
 """
 from __future__ import annotations
 
@@ -63,6 +66,10 @@ def run_single(p, precision: str, steps: int, every: int, keep_fields: bool):
     problems.load(mb, p)
     mb.preloop()
     hist = [mb.reduce()]
+    b0 = [mb.download(f) for f in ("bx", "by", "bz")]
+    bmax = max(float(np.abs(_interior(a)).max()) for a in b0)
+    d0 = problems.div_h_b(*b0)
+    del b0
     mb.run(0, 2)  # autotune and graph capture, timed apart
     mb.sync()
     done, wall = 2, 0.0
@@ -80,8 +87,10 @@ def run_single(p, precision: str, steps: int, every: int, keep_fields: bool):
         if time.time() - t_last > 20:
             log(f"  {p.name} {precision}: step {done}/{steps}")
             t_last = time.time()
+    d1 = problems.div_h_b(*(mb.download(f) for f in ("bx", "by", "bz")))
     out = {"hist": np.array(hist), "cups": p.cells * (steps - 2) / wall, "ms_per_step": wall * 1e3 / (steps - 2),
-           "tuning": mb.get_tuning()}
+           "tuning": mb.get_tuning(), "divb": problems.divb_report(p, d0, d1, bmax)}
+    del d0, d1
     if keep_fields:
         out["fields"] = {f: _interior(mb.download(f)).copy() for f in F6}
     mb.close()
@@ -122,7 +131,7 @@ def compare(p, r64, r32):
                        "l2_rel": float(np.sqrt(np.sum(d * d) / max(np.sum(a[ok] ** 2), 1e-300))),
                        "nonfinite": int((~np.isfinite(a)).sum() + (~np.isfinite(b)).sum())}
         res["fields"] = comp
-    res["divb_max"] = {"fp64": float(r64["hist"][-1, 2]), "fp32_bulk": float(r32["hist"][-1, 2])}
+    res["divb"] = {"fp64": r64["divb"], "other": r32["divb"]}
     return res
 
 
@@ -181,6 +190,11 @@ def scan_workload(count, steps, seed):
             problems.load(b[q], p)
             b[q].preloop()
         h0 = [b[q].reduce() for q in range(count)]
+        d0, bmax = [], []
+        for q in range(count):
+            bq = [b[q].download(f) for f in ("bx", "by", "bz")]
+            bmax.append(max(float(np.abs(_interior(a)).max()) for a in bq))
+            d0.append(problems.div_h_b(*bq))
         b.run(0, 2)
         t = time.perf_counter()
         b.run(2, steps - 2)
@@ -188,6 +202,12 @@ def scan_workload(count, steps, seed):
             b[q].sync()
         wall = time.perf_counter() - t
         hists[prec] = (np.array(h0), np.array([b[q].reduce() for q in range(count)]))
+        reps = [problems.divb_report(p, d0[q], problems.div_h_b(*(b[q].download(f) for f in ("bx", "by", "bz"))),
+                                     bmax[q]) for q, p in enumerate(probs)]
+        del d0
+        out[prec + "_divb"] = {
+            "change_max_free_rel_bmax_worst": max(r["change_max_free_rel_bmax"] for r in reps),
+            "max_t0_worst": max(r["max_t0"] for r in reps), "max_end_worst": max(r["max_end"] for r in reps)}
         out[prec] = {"batch_cups": cells * (steps - 2) / wall, "batch_ms_per_step": wall * 1e3 / (steps - 2),
                      "tuning": b.get_tuning()}
         b.close()
