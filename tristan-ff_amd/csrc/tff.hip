@@ -1655,6 +1655,8 @@ struct tff_batch {
   void* d_stage = nullptr;  // BatchStage[4][n]: stage 1 from a clean state, stages 1, 2, 3
   int* d_step = nullptr;
   void* d_fixmask = nullptr;  // FP32-bulk: one word per work-list tile
+  unsigned* d_queue = nullptr;  // k_batch_stage's segment counters (cleared by each launch)
+  BatchQueue qseg{};            // the work list cut into 8 segments of about equal cost
   hipGraphExec_t graph = nullptr;
 };
 
@@ -1693,8 +1695,13 @@ int batch_launch_stage_t(tff_batch* b, int which) {
   const int n = (int)b->inst.size();
   const BatchStage<S, P>* st = (const BatchStage<S, P>*)b->d_stage + (size_t)which * n;
   const unsigned grid = (unsigned)std::min<int64_t>(b->grid, b->total_tiles);
-  hipLaunchKernelGGL((k_batch_stage<PR, TL>), dim3(grid), dim3(TL::NT), lds, b->stream, st, b->d_tile_base, n,
-                     b->total_tiles);
+  BatchQueue q = b->qseg;
+  if (grid % 8 != 0) {  // one queue: segment 0 holds every item
+    for (int x = 1; x < 9; ++x) q.seg[x] = b->total_tiles;
+    q.seg[0] = 0;
+  }
+  hipLaunchKernelGGL((k_batch_stage<PR, TL>), dim3(grid), dim3(TL::NT), lds, b->stream, st, b->d_tile_base, n, q,
+                     b->d_queue);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return bfail(b, TFF_EHIP, std::string("k_batch_stage: ") + hipGetErrorString(e));
   if constexpr (PR::null_fp64) {
@@ -1752,6 +1759,7 @@ int batch_build_t(tff_batch* b) {
   std::vector<int64_t> tile_base(n), shell_base, floor_base;
   std::vector<int> shell_inst, floor_inst;
   std::vector<BatchInfo> info(n);
+  std::vector<double> cost;
   int64_t tt = 0, ss = 0, ff = 0;
   for (int q = 0; q < n; ++q) {
     tff_handle* h = b->inst[q];
@@ -1763,6 +1771,15 @@ int batch_build_t(tff_batch* b) {
     a.nblocks = a.nbx * a.nby * ((h->g.nz + a.kchunk - 1) / a.kchunk);
     tile_base[q] = tt;
     tt += a.nblocks;
+    // estimated cost of each tile: its planes plus the four warm-up planes,
+    // times the waves that hold an active cell plus about three for the
+    // cleaned halo ring
+    for (int t = 0; t < a.nblocks; ++t) {
+      const int by = (t / a.nbx) % a.nby, bz = t / (a.nbx * a.nby);
+      const int rows = std::min(TL::TY, h->g.ny - by * TL::TY);
+      const int planes = std::min(a.kchunk, h->g.nz - bz * a.kchunk);
+      cost.push_back((double)(planes + 4) * ((rows * TL::TX + 63) / 64 + 3));
+    }
     // per-instance lists padded to whole blocks of the ghost passes, so
     // that a block never spans two instances
     auto pad = [](int64_t n) { return (n + BATCH_TB - 1) / BATCH_TB * BATCH_TB; };
@@ -1828,6 +1845,21 @@ int batch_build_t(tff_batch* b) {
     info[q].bcp = h->d_bc;
   }
   b->total_tiles = tt;
+  {
+    // segment x ends at the first item whose running cost reaches (x+1)/8
+    // of the total
+    double tot = 0.0;
+    for (double c : cost) tot += c;
+    double run = 0.0;
+    int x = 0;
+    b->qseg.seg[0] = 0;
+    for (int64_t t = 0; t < tt && x < 7; ++t) {
+      run += cost[(size_t)t];
+      while (x < 7 && run >= tot * (x + 1) / 8.0) b->qseg.seg[++x] = t + 1;
+    }
+    while (x < 8) b->qseg.seg[++x] = tt;
+    b->qseg.seg[8] = tt;
+  }
   b->total_shell = ss;
   b->nghost = (int)shell_base.size();
   b->nfloor = (int)floor_base.size();
@@ -1856,6 +1888,8 @@ int batch_build_t(tff_batch* b) {
       for (int q = 0; q < n; ++q)
         st[(size_t)w * n + q].a.fixmask = (unsigned long long*)b->d_fixmask + tile_base[q];
   }
+  if (!b->d_queue) BHIP(b, hipMalloc((void**)&b->d_queue, BATCH_QWORDS * sizeof(unsigned)));
+  BHIP(b, hipMemsetAsync(b->d_queue, 0, BATCH_QWORDS * sizeof(unsigned), b->stream));
   BHIP(b, hipMalloc(&b->d_stage, st.size() * sizeof(st[0])));
   BHIP(b, hipMemcpyAsync(b->d_stage, st.data(), st.size() * sizeof(st[0]), hipMemcpyHostToDevice, b->stream));
   BHIP(b, hipMalloc((void**)&b->d_tile_base, n * sizeof(int64_t)));
@@ -2097,6 +2131,7 @@ void tff_batch_destroy(tff_batch* b) {
   hipFree(b->d_info);
   hipFree(b->d_stage);
   hipFree(b->d_fixmask);
+  hipFree(b->d_queue);
   hipFree(b->d_step);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;

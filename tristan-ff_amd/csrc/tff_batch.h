@@ -42,29 +42,59 @@ __device__ __forceinline__ int find_item(const int64_t* base, int n, int64_t t) 
   return lo;
 }
 
-// Persistent workgroups walk the (instance, tile) list with fixed strides,
-// so every item is done exactly once with no cross-workgroup
-// synchronisation and every workgroup leaves when its next item is past the
-// end.  Workgroups are dealt to the 8 XCDs round-robin (blockIdx % 8); when
-// the grid is a multiple of 8, XCD x works through the x-th contiguous
-// eighth of the list, so tiles that share halo rows share that XCD's L2.
+// The work queue of k_batch_stage.  The (instance, tile) list, in instance
+// order and x-fastest tile order within an instance, is cut into 8
+// contiguous segments of about equal estimated cost, one per XCD.
+// Workgroups are dealt to the XCDs round-robin (blockIdx % 8), so when the
+// grid is a multiple of 8 the workgroups of XCD x take the items of segment
+// x one at a time (neighbouring tiles, whose halo rows the XCD's L2 then
+// serves once), and a workgroup whose segment is exhausted takes items from
+// the next segments in turn.  Otherwise one queue holds every item.
+//
+// ctr[0..7] count the items taken from each segment, ctr[8] the workgroups
+// that have finished; the last workgroup to finish clears all nine, so each
+// launch finds them zero (the batch clears them once when it builds its
+// lists) and a captured graph needs no memset node.
+struct BatchQueue {
+  int64_t seg[9];  // segment x is items seg[x] .. seg[x+1]-1
+};
+constexpr int BATCH_QWORDS = 9;
+
 template <class PR, class TL>
 __global__ void __launch_bounds__(TL::NT) __attribute__((amdgpu_waves_per_eu(sizeof(typename PR::S) == 4 ? 4 : 2)))
     k_batch_stage(const BatchStage<typename PR::S, typename PR::P>* __restrict__ st,
-                  const int64_t* __restrict__ tile_base, int ninst, int64_t total) {
+                  const int64_t* __restrict__ tile_base, int ninst, BatchQueue qd, unsigned* ctr) {
   extern __shared__ unsigned char smem_raw[];
-  int64_t t0 = blockIdx.x, stride = gridDim.x, end = total;
-  if ((gridDim.x & 7) == 0) {
-    const int64_t chunk = (total + 7) / 8;
-    const int64_t x = blockIdx.x & 7;
-    t0 = x * chunk + (blockIdx.x >> 3);
-    stride = gridDim.x >> 3;
-    end = min(total, (x + 1) * chunk);
-  }
-  for (int64_t t = t0; t < end; t += stride) {
-    __syncthreads();  // the previous item has finished with LDS
+  __shared__ long long s_item;
+  const int nq = (gridDim.x & 7) == 0 ? 8 : 1;
+  int q = nq == 8 ? (int)(blockIdx.x & 7) : 0;  // thread 0: the segment it draws from
+  int left = nq;                                 // thread 0: segments not yet found empty
+  for (;;) {
+    __syncthreads();  // the previous item has finished with LDS and s_item
+    if (threadIdx.x == 0) {
+      long long item = -1;
+      while (left > 0) {
+        const int64_t t = qd.seg[q] + (int64_t)atomicAdd(&ctr[q], 1u);
+        if (t < qd.seg[q + 1]) {
+          item = t;
+          break;
+        }
+        --left;
+        q = q + 1 == nq ? 0 : q + 1;
+      }
+      s_item = item;
+    }
+    __syncthreads();
+    const long long t = s_item;
+    if (t < 0) break;
     const int inst = find_item(tile_base, ninst, t);
     stage_tile<PR, TL, PR::null_fp64, WRAP_RUNTIME>(st[inst].a, (int)(t - tile_base[inst]), smem_raw);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();  // every draw of this workgroup precedes its count below
+    if (atomicAdd(&ctr[8], 1u) == gridDim.x - 1) {
+      for (int x = 0; x < BATCH_QWORDS; ++x) atomicExch(&ctr[x], 0u);
+    }
   }
 }
 
