@@ -22,6 +22,10 @@ CASES = {
     "psr_faithful": lambda: problems.psr(40, radius=5.0, r_lc=12.0, absorb_buff=6, periods=0.2,
                                          absorb_fixed=False),
     "uniform": lambda: problems.uniform(24, L0=4.0, steps=12),
+    # x, y periodic and z open with a hook that reads no field: the fused
+    # path wraps x and y and stores only the z-ghost planes; an uneven box
+    # with partial tiles in x and y and fewer planes than a z-chunk
+    "uniform_flat": lambda: problems.uniform((30, 20, 6), L0=4.0, steps=12),
 }
 
 
@@ -137,7 +141,7 @@ def test_fused_fp64_matches_oracle(name):
     assert max(err.values()) <= 1e-12
 
 
-@pytest.mark.parametrize("name", ["alfven", "loop"])
+@pytest.mark.parametrize("name", ["alfven", "loop", "loop_driven", "uniform_flat"])
 def test_results_independent_of_tile_shape(name):
     p = CASES[name]()
     ref = None

@@ -470,6 +470,26 @@ bool all_periodic(const tff_handle* h) {
 // every ghost read is a periodic wrap and no hook writes ghosts
 bool wrap_mode(const tff_handle* h) { return all_periodic(h) && h->bc_kind == TFF_BC_NONE; }
 
+// A hook that reads no field value (none, the uniform twist, the driven
+// loop): it cannot see a carried value that the periodic fill overwrites.
+bool blind_hook(const tff_handle* h) {
+  return h->bc_kind == TFF_BC_NONE || h->bc_kind == TFF_BC_UNIFORM ||
+         (h->bc_kind == TFF_BC_LOOP && h->bc_host[8] == 2.0);
+}
+
+// x and y periodic, z open, a blind hook: the fused path reads the x/y
+// ghosts of the interior planes as periodic wraps and stores only the
+// z-ghost planes (WRAP_MIXED)
+bool mixed_mode(const tff_handle* h) {
+  const int* b = h->cfg.boundary;
+  return b[0] == TFF_PERIODIC && b[1] == TFF_PERIODIC && b[2] == TFF_OPEN && blind_hook(h) && !h->comp;
+}
+
+// the ghost mode of the fused stage kernel (WRAP_STORED, _PERIODIC, _MIXED)
+int ghost_mode(const tff_handle* h) {
+  return wrap_mode(h) ? WRAP_PERIODIC : (mixed_mode(h) ? WRAP_MIXED : WRAP_STORED);
+}
+
 int64_t shell_count(const tff_handle* h) {
   const int sx = h->g.nx + 2 * NG, sy = h->g.ny + 2 * NG;
   return (int64_t)2 * NG * sx * sy + (int64_t)2 * NG * sx * h->g.nz + (int64_t)2 * NG * h->g.ny * h->g.nz;
@@ -495,8 +515,11 @@ int launch_stage_t(tff_handle* h, const StageArgs<typename PR::S, typename PR::P
   if (PR::null_fp64 && (!b.fixmask || b.nblocks > h->fix_cap))
     return fail(h, TFF_ESTATE, "near-null plane masks missing or too small");
   const unsigned grid = (unsigned)((b.nblocks + 7) / 8 * 8);
-  if (b.wrap)
+  static bool attr_m = false;
+  if (b.wrap == WRAP_PERIODIC)
     launch_lds(k_stage<PR, TL, WRAP_PERIODIC>, attr_p, grid, TL::NT, lds, h->stream, b);
+  else if (b.wrap == WRAP_MIXED)
+    launch_lds(k_stage<PR, TL, WRAP_MIXED>, attr_m, grid, TL::NT, lds, h->stream, b);
   else
     launch_lds(k_stage<PR, TL, WRAP_STORED>, attr_s, grid, TL::NT, lds, h->stream, b);
   return last_launch(h, "k_stage");
@@ -512,8 +535,11 @@ int launch_fix_t(tff_handle* h, const StageArgs<typename PR::S, typename PR::P>&
   static bool attr_p = false, attr_s = false;
   const auto b = with_tile_grid<TL>(h, a);
   const unsigned grid = (unsigned)std::min<int64_t>(FIX_GRID, b.nblocks);
-  if (b.wrap)
+  static bool attr_m = false;
+  if (b.wrap == WRAP_PERIODIC)
     launch_lds(k_stage_fix<PR, TL, WRAP_PERIODIC>, attr_p, grid, TL::NT, lds, h->stream, b);
+  else if (b.wrap == WRAP_MIXED)
+    launch_lds(k_stage_fix<PR, TL, WRAP_MIXED>, attr_m, grid, TL::NT, lds, h->stream, b);
   else
     launch_lds(k_stage_fix<PR, TL, WRAP_STORED>, attr_s, grid, TL::NT, lds, h->stream, b);
   return last_launch(h, "k_stage_fix");
@@ -559,7 +585,7 @@ StageArgs<typename PR::S, typename PR::P> stage_common(tff_handle* h) {
   a.t0 = base_c<S>(h->t0);
   a.has_t0 = h->has_t0;
   a.guard = h->cfg.absorb_fixed;
-  a.wrap = wrap_mode(h) ? 1 : 0;
+  a.wrap = ghost_mode(h);
   a.kchunk = h->kchunk;
   a.fixmask = h->d_fixmask;
   return a;
@@ -597,13 +623,11 @@ int ghost_passes(tff_handle* h, const Vec& ge_in, const Vec& prev_b, const Vec& 
   const int64_t n = shell_count(h);
   const int tb = 256;
   const unsigned nb = (unsigned)((n + tb - 1) / tb);
-  // a hook that reads no field (none, the uniform twist, the driven loop)
-  // cannot see a carried value that the periodic fill then overwrites
+  // a blind hook cannot see a carried value that the periodic fill then
+  // overwrites
   const int* bd = h->cfg.boundary;
-  const bool blind = h->bc_kind == TFF_BC_NONE || h->bc_kind == TFF_BC_UNIFORM ||
-                     (h->bc_kind == TFF_BC_LOOP && h->bc_host[8] == 2.0);
-  const int3 per = blind ? int3{bd[0] == TFF_PERIODIC, bd[1] == TFF_PERIODIC, bd[2] == TFF_PERIODIC}
-                         : int3{0, 0, 0};
+  const int3 per = blind_hook(h) ? int3{bd[0] == TFF_PERIODIC, bd[1] == TFF_PERIODIC, bd[2] == TFF_PERIODIC}
+                                 : int3{0, 0, 0};
   int rc;
   {
     PhaseScope phase(h, TFF_PHASE_EXCHANGE);
@@ -622,6 +646,37 @@ int ghost_passes(tff_handle* h, const Vec& ge_in, const Vec& prev_b, const Vec& 
   return last_launch(h, "k_ghost_clean");
 }
 
+// The ghost passes of a WRAP_MIXED box after a fused stage: only the
+// z-ghost planes are stored.  They start as the stage input's values there
+// (rk3Update leaves ghosts alone; nothing cleans or fills those planes, so
+// the input's raw E is its cleaned E there), then the hook writes them, and
+// k_edge_clean stores the cleaned E of the x/y ghost ring of planes 0 and
+// nz-1 for the next stage.  No fill: the stage kernel reads the interior
+// planes' x/y ghosts as wraps, and exchangeFields leaves the edges of the
+// open z axis alone.
+template <class PR>
+int ghost_passes_mixed(tff_handle* h, const Vec& ye_in, const Vec& prev_b, const Vec& oe, const Vec& ob,
+                       const int* step_ptr, int step_val) {
+  using S = typename PR::S;
+  // the z-ghost planes come first in shell_cell's order
+  const int64_t nzc = (int64_t)2 * NG * (h->g.nx + 2 * NG) * (h->g.ny + 2 * NG);
+  const int tb = 256;
+  int rc;
+  {
+    PhaseScope phase(h, TFF_PHASE_EXCHANGE);
+    hipLaunchKernelGGL(k_carry<S>, dim3((unsigned)((nzc + tb - 1) / tb)), dim3(tb), 0, h->stream, h->g,
+                       cv3<S>(ye_in), cv3<S>(prev_b), mv3<S>(oe), mv3<S>(ob), nzc, int3{1, 1, 0});
+    rc = last_launch(h, "k_carry");
+  }
+  if (rc) return rc;
+  if ((rc = launch_bc<PR>(h, oe, ob, step_ptr, step_val))) return rc;
+  PhaseScope phase(h, TFF_PHASE_EXCHANGE);
+  const int ne = (2 * (h->g.nx + 2) + 2 * h->g.ny) * (h->g.nz > 1 ? 2 : 1);
+  hipLaunchKernelGGL(k_edge_clean<PR>, dim3((unsigned)((ne + tb - 1) / tb)), dim3(tb), 0, h->stream, h->g,
+                     base_c<S>(oe), base_c<S>(ob), base_m<S>(h->ge), h->cfg.absorb_fixed);
+  return last_launch(h, "k_edge_clean");
+}
+
 // One timestep on the fused path.  Buffers: es/bs hold the state (raw
 // output of the last stage unless state_clean), w1/w2 the stage outputs,
 // ce the cleaned U^n E, ge the cleaned ghost E.
@@ -630,10 +685,17 @@ int enqueue_fused_step(tff_handle* h, const int* step_ptr, int step_val) {
   using S = typename PR::S;
   if (!h->ge.p[0] || h->tile < 0 || h->tile >= NTILES || h->kchunk < 1)
     return fail(h, TFF_ESTATE, "fused path not set up (ghost buffer or tuning missing)");
-  const bool wrap = wrap_mode(h);
+  const bool wrap = wrap_mode(h), mixed = mixed_mode(h);
   const bool first = h->state_clean;
   int rc;
   auto a = stage_common<PR>(h);
+  // the ghost passes after a stage that read (ye, yb) and wrote (oe, ob);
+  // ge_in: the cleaned E of the input's ghosts (stored mode)
+  auto ghosts = [&](const Vec& ye, const Vec& ge_in, const Vec& yb, const Vec& oe, const Vec& ob) {
+    if (wrap) return (int)TFF_OK;
+    if (mixed) return ghost_passes_mixed<PR>(h, ye, yb, oe, ob, step_ptr, step_val);
+    return ghost_passes<PR>(h, ge_in, yb, oe, ob, step_ptr, step_val);
+  };
   // stage 1: U1 = Un + L(Un), Un = clean(state) (or the state itself when it
   // is already clean)
   set_coef<PR>(h, a, 1.0, 0.0, 1.0);
@@ -647,8 +709,7 @@ int enqueue_fused_step(tff_handle* h, const int* step_ptr, int step_val) {
   a.u0e = first ? nullptr : base_m<S>(h->ce);
   if ((rc = launch_stage<PR>(h, a, h->tile))) return rc;
   const Vec& u0e = first ? h->es : h->ce;
-  if (!wrap && (rc = ghost_passes<PR>(h, first ? h->es : h->ge, h->bs, h->w1e, h->w1b, step_ptr, step_val)))
-    return rc;
+  if ((rc = ghosts(h->es, first ? h->es : h->ge, h->bs, h->w1e, h->w1b))) return rc;
   // stage 2: U2 = 3/4 Un + 1/4 X1 + 1/4 L(X1)
   set_coef<PR>(h, a, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0);
   a.ye = base_c<S>(h->w1e);
@@ -662,7 +723,7 @@ int enqueue_fused_step(tff_handle* h, const int* step_ptr, int step_val) {
   a.ob = base_m<S>(h->w2b);
   a.u0e = nullptr;
   if ((rc = launch_stage<PR>(h, a, h->tile))) return rc;
-  if (!wrap && (rc = ghost_passes<PR>(h, h->ge, h->w1b, h->w2e, h->w2b, step_ptr, step_val))) return rc;
+  if ((rc = ghosts(h->w1e, h->ge, h->w1b, h->w2e, h->w2b))) return rc;
   // stage 3: U3 = 1/3 Un + 2/3 X2 + 2/3 L(X2), written over the state in
   // place (each thread reads Un only at its own cell before writing it)
   set_coef<PR>(h, a, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0);
@@ -671,7 +732,7 @@ int enqueue_fused_step(tff_handle* h, const int* step_ptr, int step_val) {
   a.oe = base_m<S>(h->es);
   a.ob = base_m<S>(h->bs);
   if ((rc = launch_stage<PR>(h, a, h->tile))) return rc;
-  if (!wrap && (rc = ghost_passes<PR>(h, h->ge, h->w2b, h->es, h->bs, step_ptr, step_val))) return rc;
+  if ((rc = ghosts(h->w2e, h->ge, h->w2b, h->es, h->bs))) return rc;
   h->state_clean = false;
   return TFF_OK;
 }
@@ -684,11 +745,24 @@ int ensure_clean_t(tff_handle* h) {
   using S = typename PR::S;
   if (h->state_clean) return TFF_OK;
   int rc;
-  // in wrap mode the raw state's ghosts were never written: fill them first
-  // (the reference's fill before cleanEpar), then clean and fill again
-  if (wrap_mode(h) && (rc = launch_fill<S>(h, {&h->es, &h->bs}))) return rc;
+  // in wrap mode the raw state's ghosts were never written, in mixed mode
+  // its x/y ghosts of the interior planes: fill them first (the reference's
+  // fill before cleanEpar), then clean and fill again
+  const bool wrap = wrap_mode(h), mixed = mixed_mode(h);
+  if ((wrap || mixed) && (rc = launch_fill<S>(h, {&h->es, &h->bs}))) return rc;
   if ((rc = launch_clean<PR>(h, h->es, h->bs, h->ce))) return rc;
-  if (wrap_mode(h)) {
+  if (wrap) {
+    if ((rc = launch_fill<S>(h, {&h->ce}))) return rc;
+  } else if (mixed) {
+    // the z-ghost planes (edges included) are never cleaned: every ghost
+    // takes the raw value, then the fill puts the cleaned wraps on the x/y
+    // ghosts of the interior planes
+    const int64_t n = shell_count(h);
+    const int tb = 256;
+    hipLaunchKernelGGL(k_carry<S>, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, h->stream, h->g,
+                       cv3<S>(h->es), cv3<S>(h->bs), mv3<S>(h->ce), V3<S>{nullptr, nullptr, nullptr}, n,
+                       int3{0, 0, 0});
+    if ((rc = last_launch(h, "k_carry"))) return rc;
     if ((rc = launch_fill<S>(h, {&h->ce}))) return rc;
   } else {
     const int64_t n = shell_count(h);
@@ -1647,6 +1721,16 @@ struct tff_batch {
   int64_t* d_tile_base = nullptr;
   int64_t* d_shell_base = nullptr;
   int* d_shell_inst = nullptr;
+  // ghost-clean list (stored-ghost instances) and edge-ring list (WRAP_MIXED
+  // instances); the shell list above feeds the build pass of both kinds
+  int64_t* d_clean_base = nullptr;
+  int* d_clean_inst = nullptr;
+  int nclean = 0;
+  int64_t total_clean = 0;
+  int64_t* d_edge_base = nullptr;
+  int* d_edge_inst = nullptr;
+  int nedge = 0;
+  int64_t total_edge = 0;
   int64_t* d_floor_base = nullptr;  // driven-loop instances: interior floor planes
   int* d_floor_inst = nullptr;
   int nfloor = 0;
@@ -1756,15 +1840,15 @@ int batch_build_t(tff_batch* b) {
   using P = typename PR::P;
   const int n = (int)b->inst.size();
   std::vector<BatchStage<S, P>> st(4 * (size_t)n);
-  std::vector<int64_t> tile_base(n), shell_base, floor_base;
-  std::vector<int> shell_inst, floor_inst;
+  std::vector<int64_t> tile_base(n), shell_base, floor_base, clean_base, edge_base;
+  std::vector<int> shell_inst, floor_inst, clean_inst, edge_inst;
   std::vector<BatchInfo> info(n);
   std::vector<double> cost;
-  int64_t tt = 0, ss = 0, ff = 0;
+  int64_t tt = 0, ss = 0, ff = 0, cl = 0, ee = 0;
   for (int q = 0; q < n; ++q) {
     tff_handle* h = b->inst[q];
     h->kchunk = b->kchunk;
-    const bool wrap = wrap_mode(h);
+    const bool wrap = wrap_mode(h), mixed = mixed_mode(h);
     auto a = stage_common<PR>(h);
     a.nbx = (h->g.nx + TL::TX - 1) / TL::TX;
     a.nby = (h->g.ny + TL::TY - 1) / TL::TY;
@@ -1783,10 +1867,24 @@ int batch_build_t(tff_batch* b) {
     // per-instance lists padded to whole blocks of the ghost passes, so
     // that a block never spans two instances
     auto pad = [](int64_t n) { return (n + BATCH_TB - 1) / BATCH_TB * BATCH_TB; };
+    info[q].nbuild = 0;
     if (!wrap) {
+      // a WRAP_MIXED instance stores only its z-ghost planes (the first
+      // cells in shell_cell's order) and cleans only its edge rings
+      const int64_t nb = mixed ? (int64_t)2 * NG * (h->g.nx + 2 * NG) * (h->g.ny + 2 * NG) : shell_count(h);
+      info[q].nbuild = nb;
       shell_base.push_back(ss);
       shell_inst.push_back(q);
-      ss += pad(shell_count(h));
+      ss += pad(nb);
+      if (mixed) {
+        edge_base.push_back(ee);
+        edge_inst.push_back(q);
+        ee += pad((int64_t)(2 * (h->g.nx + 2) + 2 * h->g.ny) * (h->g.nz > 1 ? 2 : 1));
+      } else {
+        clean_base.push_back(cl);
+        clean_inst.push_back(q);
+        cl += pad(nb);
+      }
     }
     if (h->bc_kind == TFF_BC_LOOP && h->bc_host[8] == 2.0) {
       floor_base.push_back(ff);
@@ -1837,6 +1935,9 @@ int batch_build_t(tff_batch* b) {
           x.prev_b = base_c<S>(h->w2b);
         }
       }
+      // a WRAP_MIXED instance's carried z-ghost E is the stage input's raw
+      // E (nothing cleans those planes)
+      if (mixed) x.ge_in = x.a.ye;
     }
     info[q].per[0] = h->cfg.boundary[0] == TFF_PERIODIC;
     info[q].per[1] = h->cfg.boundary[1] == TFF_PERIODIC;
@@ -1862,6 +1963,10 @@ int batch_build_t(tff_batch* b) {
   }
   b->total_shell = ss;
   b->nghost = (int)shell_base.size();
+  b->nclean = (int)clean_base.size();
+  b->total_clean = cl;
+  b->nedge = (int)edge_base.size();
+  b->total_edge = ee;
   b->nfloor = (int)floor_base.size();
   b->total_floor = ff;
   // the batch stream is non-blocking: the old lists may only be freed, and
@@ -1871,10 +1976,16 @@ int batch_build_t(tff_batch* b) {
   hipFree(b->d_tile_base);
   hipFree(b->d_shell_base);
   hipFree(b->d_shell_inst);
+  hipFree(b->d_clean_base);
+  hipFree(b->d_clean_inst);
+  hipFree(b->d_edge_base);
+  hipFree(b->d_edge_inst);
   hipFree(b->d_floor_base);
   hipFree(b->d_floor_inst);
   hipFree(b->d_info);
   hipFree(b->d_fixmask);
+  b->d_clean_base = b->d_edge_base = nullptr;
+  b->d_clean_inst = b->d_edge_inst = nullptr;
   b->d_fixmask = nullptr;
   b->d_floor_base = nullptr;
   b->d_floor_inst = nullptr;
@@ -1901,6 +2012,22 @@ int batch_build_t(tff_batch* b) {
     BHIP(b, hipMemcpyAsync(b->d_shell_base, shell_base.data(), shell_base.size() * sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipMalloc((void**)&b->d_shell_inst, shell_inst.size() * sizeof(int)));
     BHIP(b, hipMemcpyAsync(b->d_shell_inst, shell_inst.data(), shell_inst.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+  }
+  if (b->nclean) {
+    BHIP(b, hipMalloc((void**)&b->d_clean_base, clean_base.size() * sizeof(int64_t)));
+    BHIP(b, hipMemcpyAsync(b->d_clean_base, clean_base.data(), clean_base.size() * sizeof(int64_t),
+                           hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMalloc((void**)&b->d_clean_inst, clean_inst.size() * sizeof(int)));
+    BHIP(b, hipMemcpyAsync(b->d_clean_inst, clean_inst.data(), clean_inst.size() * sizeof(int),
+                           hipMemcpyHostToDevice, b->stream));
+  }
+  if (b->nedge) {
+    BHIP(b, hipMalloc((void**)&b->d_edge_base, edge_base.size() * sizeof(int64_t)));
+    BHIP(b, hipMemcpyAsync(b->d_edge_base, edge_base.data(), edge_base.size() * sizeof(int64_t),
+                           hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMalloc((void**)&b->d_edge_inst, edge_inst.size() * sizeof(int)));
+    BHIP(b, hipMemcpyAsync(b->d_edge_inst, edge_inst.data(), edge_inst.size() * sizeof(int),
+                           hipMemcpyHostToDevice, b->stream));
   }
   if (b->nfloor) {
     BHIP(b, hipMalloc((void**)&b->d_floor_base, floor_base.size() * sizeof(int64_t)));
@@ -1934,8 +2061,13 @@ int batch_ghosts(tff_batch* b, int which) {
   const unsigned nb = (unsigned)((b->total_shell + tb - 1) / tb);
   hipLaunchKernelGGL(k_batch_ghost_build<PR>, dim3(nb), dim3(tb), 0, b->stream, st, (const BatchInfo*)b->d_info,
                      b->d_shell_base, b->d_shell_inst, b->nghost, b->total_shell, b->d_step);
-  hipLaunchKernelGGL(k_batch_ghost_clean<PR>, dim3(nb), dim3(tb), 0, b->stream, st, (const BatchInfo*)b->d_info,
-                     b->d_shell_base, b->d_shell_inst, b->nghost, b->total_shell);
+  if (b->nclean)
+    hipLaunchKernelGGL(k_batch_ghost_clean<PR>, dim3((unsigned)((b->total_clean + tb - 1) / tb)), dim3(tb), 0,
+                       b->stream, st, (const BatchInfo*)b->d_info, b->d_clean_base, b->d_clean_inst, b->nclean,
+                       b->total_clean);
+  if (b->nedge)
+    hipLaunchKernelGGL(k_batch_edge_clean<PR>, dim3((unsigned)((b->total_edge + tb - 1) / tb)), dim3(tb), 0,
+                       b->stream, st, b->d_edge_base, b->d_edge_inst, b->nedge, b->total_edge);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return bfail(b, TFF_EHIP, std::string("batch ghost passes: ") + hipGetErrorString(e));
   return TFF_OK;
@@ -2126,6 +2258,10 @@ void tff_batch_destroy(tff_batch* b) {
   hipFree(b->d_tile_base);
   hipFree(b->d_shell_base);
   hipFree(b->d_shell_inst);
+  hipFree(b->d_clean_base);
+  hipFree(b->d_clean_inst);
+  hipFree(b->d_edge_base);
+  hipFree(b->d_edge_inst);
   hipFree(b->d_floor_base);
   hipFree(b->d_floor_inst);
   hipFree(b->d_info);

@@ -29,6 +29,8 @@ struct BatchInfo {
   int per[3];         // periodic axes
   int bc_kind;        // TFF_BC_NONE (0) or TFF_BC_LOOP (1)
   const double* bcp;  // hook parameters (device)
+  int64_t nbuild;     // ghost cells of the build pass: the whole shell, or
+                      // the z-ghost planes of a WRAP_MIXED instance
 };
 
 // index of the last entry of base[0..n) that is <= t
@@ -131,7 +133,7 @@ __global__ void k_batch_ghost_build(const BatchStage<typename PR::S, typename PR
   const Geom& g = b.a.g;
   const int64_t cs = g.cs;
   int i, j, k;
-  if (!shell_cell(g, t - shell_base[q], i, j, k)) return;
+  if (t - shell_base[q] >= in.nbuild || !shell_cell(g, t - shell_base[q], i, j, k)) return;
   const int64_t d = g.at(i, j, k);
   S* oe = b.a.oe;
   S* ob = b.a.ob;
@@ -263,6 +265,20 @@ __global__ void k_batch_ghost_clean(const BatchStage<typename PR::S, typename PR
   b.ge_out[d] = (S)px;
   b.ge_out[d + cs] = (S)py;
   b.ge_out[d + 2 * cs] = (S)pz;
+}
+
+// Cleaned E of the x/y ghost ring of planes 0 and nz-1 of every WRAP_MIXED
+// instance (k_edge_clean for the batch).
+template <class PR>
+__global__ void k_batch_edge_clean(const BatchStage<typename PR::S, typename PR::P>* st, const int64_t* edge_base,
+                                   const int* edge_inst, int nedge, int64_t total) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int q = find_item(edge_base, nedge, (int64_t)blockIdx.x * blockDim.x);
+  const auto& b = st[edge_inst[q]];
+  const int r = (int)(t - edge_base[q]);
+  if (r >= edge_ring_cells(b.a.g)) return;
+  edge_clean_at<PR>(b.a.g, b.a.oe, b.a.ob, b.ge_out, b.a.guard, r);
 }
 
 }  // namespace tff

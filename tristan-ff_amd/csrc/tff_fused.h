@@ -173,10 +173,20 @@ __device__ __forceinline__ int slot(int p) {
 }
 
 // Ghost-cell handling of a stage launch, fixed at compile time where the
-// launch knows it: WRAP_STORED (ghosts are stored), WRAP_PERIODIC (every
-// axis periodic, no ghost is read or written), WRAP_RUNTIME (A.wrap
-// decides; the batched kernel mixes both kinds of instances).
-constexpr int WRAP_STORED = 0, WRAP_PERIODIC = 1, WRAP_RUNTIME = 2;
+// launch knows it (A.wrap holds the same value):
+//   WRAP_STORED    ghosts are stored: raw reads take them, the cleaned E of
+//                  a ghost cell comes from `ge`;
+//   WRAP_PERIODIC  every axis periodic: halo reads wrap around the box and
+//                  no ghost is read or written;
+//   WRAP_MIXED     x and y periodic, z open with a hook that reads no field:
+//                  reads in the interior planes wrap in x and y (the values
+//                  exchangeFields would copy there), the z-ghost planes are
+//                  stored and never cleaned (their cleaned E is the raw E),
+//                  and the x/y ghosts of planes 0 and nz-1, whose cleaned
+//                  values depend on stored z-ghost edges, come from `ge`
+//                  (k_edge_clean);
+//   WRAP_RUNTIME   A.wrap decides (the batched kernel mixes instances).
+constexpr int WRAP_STORED = 0, WRAP_PERIODIC = 1, WRAP_MIXED = 2, WRAP_RUNTIME = 3;
 
 // Planes k0..k1-1 of the TX x TY column at (i0, j0) of a stage.
 //
@@ -214,25 +224,28 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
   const Geom& g = A.g;
   const int64_t cs = g.cs;
   const int tid = threadIdx.x;
-  const bool wrap = WM == WRAP_RUNTIME ? A.wrap != 0 : WM == WRAP_PERIODIC;
+  const int mode = WM == WRAP_RUNTIME ? A.wrap : WM;
+  const bool wrap = mode == WRAP_PERIODIC;  // every axis wraps
+  const bool mixed = mode == WRAP_MIXED;    // x and y wrap, z stored
   if (FIX && tid < RG::FLAGS) nflag[tid] = 0u;
 
   // x-y byte offsets (from the plane origin) of this thread's halo cells
-  // of the raw planes: wrapped on periodic boxes, clamped to the stored
-  // ghosts otherwise
-  uint32_t hxy[L2PT];
+  // of the raw planes: wrapped in x and y on periodic and mixed boxes,
+  // clamped to the stored ghosts otherwise; `hxs` (mixed boxes) the
+  // clamped ones, for the stored z-ghost planes
+  uint32_t hxy[L2PT], hxs[L2PT];
 #pragma unroll
   for (int m = 0; m < L2PT; ++m) {
     const int c = min(tid + m * NT, N2 - 1);
-    int gi = i0 - 2 + c % W2, gj = j0 - 2 + c / W2;
-    if (wrap) {
-      gi = wrap_index(gi, g.nx);
-      gj = wrap_index(gj, g.ny);
+    const int ri = i0 - 2 + c % W2, rj = j0 - 2 + c / W2;
+    const int si = min(ri, g.nx + NG - 1), sj = min(rj, g.ny + NG - 1);
+    hxs[m] = (uint32_t)((g.org + si + (int64_t)sj * g.px) * (int64_t)sizeof(S));
+    if (wrap || mixed) {
+      const int wi = wrap_index(ri, g.nx), wj = wrap_index(rj, g.ny);
+      hxy[m] = (uint32_t)((g.org + wi + (int64_t)wj * g.px) * (int64_t)sizeof(S));
     } else {
-      gi = min(gi, g.nx + NG - 1);
-      gj = min(gj, g.ny + NG - 1);
+      hxy[m] = hxs[m];
     }
-    hxy[m] = (uint32_t)((g.org + gi + (int64_t)gj * g.px) * (int64_t)sizeof(S));
   }
   // uniform pointer to plane p of a buffer's component 0
   auto plane_of = [&](const S* base, int p) -> const S* {
@@ -251,10 +264,11 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
     const S* e2 = e0 + 2 * cs;
     const S* b1 = b0 + cs;
     const S* b2 = b0 + 2 * cs;
+    const bool stored_plane = mixed && (p < 0 || p >= g.nz);
 #pragma unroll
     for (int m = 0; m < L2PT; ++m) {
       if (tid + m * NT < N2) {
-        const uint32_t o = hxy[m];
+        const uint32_t o = stored_plane ? hxs[m] : hxy[m];
         reg[m][0] = ld_at(e0, o);
         reg[m][1] = ld_at(e1, o);
         reg[m][2] = ld_at(e2, o);
@@ -320,8 +334,12 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
     auto BYF = [&](int a, int b, int d) { return (T)lby(a, b, d); };
     auto BZF = [&](int a, int b, int d) { return (T)lbz(a, b, d); };
     S ox, oy, oz;
-    const bool ghost = !wrap && (gi < 0 || gi >= g.nx || gj < 0 || gj >= g.ny || q < 0 || q >= g.nz);
-    const bool project = A.clean_in && !ghost;
+    const bool xyo = gi < 0 || gi >= g.nx || gj < 0 || gj >= g.ny, zo = q < 0 || q >= g.nz;
+    // ghost: the cleaned value is stored in `ge`; raw: it is the raw value
+    // (a mixed box's z-ghost planes, which nothing cleans)
+    const bool ghost = mixed ? (!zo && xyo && (q == 0 || q == g.nz - 1)) : (!wrap && (xyo || zo));
+    const bool raw = mixed && zo;
+    const bool project = A.clean_in && !ghost && !raw;
     BBar<Q> bb{};
     Q bref = (Q)0;
     if (project || OWN) {
@@ -329,7 +347,7 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
       if (PR::null_fp64) bref = cell_b2<Q>(BXF, BYF, BZF);
       if (FIX && near_null_cell<Q>(bb, bref)) nflag[(unsigned)q & FM] = 1u;
     }
-    if (!A.clean_in) {
+    if (!A.clean_in || raw) {
       ox = lex(0, 0, 0);
       oy = ley(0, 0, 0);
       oz = lez(0, 0, 0);
@@ -363,6 +381,10 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
   // own cell, then the first RING threads the halo ring
   // (own_tag false: the own cell's values are not needed)
   auto clean_plane = [&](int q, int qs, auto own_tag, Own& dst) {
+    // the waves that also clean the halo ring reach the next barrier last:
+    // they issue first while they clean
+    const bool ring_wave = (tid & ~63) < RING;
+    if (ring_wave) __builtin_amdgcn_s_setprio(2);
     clean_at(q, qs, tid % TX + 1, tid / TX + 1, own_tag, dst);
     for (int r = tid; r < RING; r += NT) {
       int mx, my;
@@ -381,6 +403,7 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
       }
       clean_at(q, qs, mx, my, std::false_type{}, dst);
     }
+    if (ring_wave) __builtin_amdgcn_s_setprio(0);
   };
 
   // prologue: planes k0-2 .. k0+1 raw, k0-1 and k0 cleaned (k0's own-cell
@@ -651,6 +674,69 @@ __global__ void k_ghost_clean(Geom g, int px_, int py_, int pz_, V3<const typena
   ge.x[d] = (S)px;
   ge.y[d] = (S)py;
   ge.z[d] = (S)pz;
+}
+
+// Cleaned E of the x/y ghost cells of planes 0 and nz-1 of a WRAP_MIXED
+// box (the 1-cell ring around each plane), into `ge`.  exchangeFields
+// copies there the cleaned value of the wrapped interior cell, whose
+// stencil reaches into the z-ghost planes: its neighbours in the interior
+// planes are read wrapped in x and y (the copies exchangeFields makes),
+// those in a z-ghost plane at their stored place (exchangeFields leaves the
+// edges of an open axis alone).  Buffers are component-0 bases, components
+// g.cs apart; t numbers the ring cells of plane 0, then of plane nz-1.
+__device__ __forceinline__ int edge_ring_cells(const Geom& g) {
+  return (2 * (g.nx + 2) + 2 * g.ny) * (g.nz > 1 ? 2 : 1);
+}
+template <class PR>
+__device__ __forceinline__ void edge_clean_at(const Geom& g, const typename PR::S* ye, const typename PR::S* yb,
+                                              typename PR::S* ge, int guard, int t) {
+  using S = typename PR::S;
+  using T = typename PR::T;
+  using P = typename PR::P;
+  const int ring = 2 * (g.nx + 2) + 2 * g.ny;
+  const int q = t < ring ? 0 : g.nz - 1;
+  const int r = t < ring ? t : t - ring;
+  int i, j;
+  if (r < g.nx + 2) {
+    i = r - 1;
+    j = -1;
+  } else if (r < 2 * (g.nx + 2)) {
+    i = r - (g.nx + 2) - 1;
+    j = g.ny;
+  } else if (r < 2 * (g.nx + 2) + g.ny) {
+    i = -1;
+    j = r - 2 * (g.nx + 2);
+  } else {
+    i = g.nx;
+    j = r - 2 * (g.nx + 2) - g.ny;
+  }
+  const int wi = wrap_index(i, g.nx), wj = wrap_index(j, g.ny);
+  const int64_t cs = g.cs;
+  auto at = [&](int a, int b, int d) -> int64_t {
+    const int k = q + d;
+    if (k >= 0 && k < g.nz) return g.at(wrap_index(wi + a, g.nx), wrap_index(wj + b, g.ny), k);
+    return g.at(wi + a, wj + b, k);
+  };
+  auto EX = [&](int a, int b, int d) { return (T)ye[at(a, b, d)]; };
+  auto EY = [&](int a, int b, int d) { return (T)ye[at(a, b, d) + cs]; };
+  auto EZ = [&](int a, int b, int d) { return (T)ye[at(a, b, d) + 2 * cs]; };
+  auto BXF = [&](int a, int b, int d) { return (T)yb[at(a, b, d)]; };
+  auto BYF = [&](int a, int b, int d) { return (T)yb[at(a, b, d) + cs]; };
+  auto BZF = [&](int a, int b, int d) { return (T)yb[at(a, b, d) + 2 * cs]; };
+  P px, py, pz;
+  clean_cell<PR>(EX, EY, EZ, BXF, BYF, BZF, guard, px, py, pz);
+  const int64_t d = g.at(i, j, q);
+  ge[d] = (S)px;
+  ge[d + cs] = (S)py;
+  ge[d + 2 * cs] = (S)pz;
+}
+
+template <class PR>
+__global__ void k_edge_clean(Geom g, const typename PR::S* ye, const typename PR::S* yb, typename PR::S* ge,
+                             int guard) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= edge_ring_cells(g)) return;
+  edge_clean_at<PR>(g, ye, yb, ge, guard, t);
 }
 
 }  // namespace tff
