@@ -17,6 +17,10 @@ steps from the same initial state:
   reductions (the headline ``value``);
 * fp64      -- FP64 throughout (parity-checked against the FP64 oracle).
 
+A third workload, the driven twisted loop at 384^3 (x/y periodic, z open,
+the footpoint driver on the floor and a conducting top), is measured the
+same way.
+
 The FP32-bulk energy is compared with the FP64 path's energy after the run.
 ``roofline`` is for the dominant kernel (the stage kernel of the FP32-bulk
 path): algorithmic bytes per launch over its average launch time, the time
@@ -195,12 +199,24 @@ def load_traffic(workload: str, source_hash: str):
     return tr, f"profiles/traffic_{workload}.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-kernel calibrated)"
 
 
-def measure_workload(name: str, prob, steps: int, warmup: int, timing_steps: int, pinned: bool) -> dict:
-    """Both paths on one workload, FP32-bulk's energy against FP64's."""
-    tuning = {"fp64": (1, 64), "fp32_bulk": (3, 64)}  # tile index (32x8, 64x8), z-chunk
+# Stage-kernel shapes the benchmarks pin, (tile index, z-chunk) per path:
+# the autotuner's usual pick on each workload, so that the line and a
+# profile of the same command time the same kernel.  Tile indices: 0 32x4,
+# 1 32x8, 2 64x4, 3 64x8, 4 32x16, 5 64x16.
+PINNED = {
+    "alfven": {"fp64": (3, 64), "fp32_bulk": (3, 64)},
+    "psr": {"fp64": (1, 32), "fp32_bulk": (4, 32)},
+    "loop": {"fp64": (3, 64), "fp32_bulk": (4, 16)},
+}
+
+
+def measure_workload(name: str, prob, steps: int, warmup: int, timing_steps: int, tuning) -> dict:
+    """Both paths on one workload, FP32-bulk's energy against FP64's;
+    `tuning` maps each path to its pinned (tile, z-chunk), or is None (the
+    autotuner)."""
     res = {}
     for prec in ("fp64", "fp32_bulk"):
-        res[prec] = run_path(prec, prob, steps, warmup, timing_steps, tuning[prec] if pinned else None)
+        res[prec] = run_path(prec, prob, steps, warmup, timing_steps, tuning[prec] if tuning else None)
         print(f"[bench] {name} {prec}: {res[prec]['value']:.4g} cell-updates/s, "
               f"{res[prec]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     e64, e32 = res["fp64"]["energy_end"], res["fp32_bulk"]["energy_end"]
@@ -264,13 +280,14 @@ def main(argv=None) -> int:
     ap.add_argument("--timing-steps", type=int, default=20)
     ap.add_argument("--psr-n", type=int, default=512, help="rotator box (0: leave the rotator out)")
     ap.add_argument("--psr-steps", type=int, default=200)
+    ap.add_argument("--loop-n", type=int, default=384, help="driven-loop box (0: leave the loop out)")
+    ap.add_argument("--loop-steps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU work per baseline process")
     ap.add_argument("--tuning", default="pinned", choices=("pinned", "auto"),
                     help="pinned: the stage kernel's tile and z-chunk that the autotuner picks on "
-                         "the Alfvenic workload in most runs (FP32-bulk 64x8 tiles, FP64 32x8, z-chunk 64), "
-                         "so that the line and a profile of the same command time the same kernel; "
-                         "auto: the autotuner")
+                         "each workload in most runs (PINNED), so that the line and a profile of the "
+                         "same command time the same kernel; auto: the autotuner")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: the force-free step runs on one GPU only (asked for --gpus {a.gpus}); "
@@ -289,7 +306,8 @@ def main(argv=None) -> int:
     t0 = time.time()
     prob = problems.alfven(a.n, seed=1, kmax=8, host=False)
     workload = f"alfven{a.n}"
-    res = measure_workload(workload, prob, a.steps, a.warmup, a.timing_steps, a.tuning == "pinned")
+    pin = a.tuning == "pinned"
+    res = measure_workload(workload, prob, a.steps, a.warmup, a.timing_steps, PINNED["alfven"] if pin else None)
     print(f"[bench] {workload} done in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     head = res["fp32_bulk"]
     main_sum = summary(res, head, workload)
@@ -316,14 +334,27 @@ def main(argv=None) -> int:
         t0 = time.time()
         psr = problems.psr(a.psr_n, host=False)
         wl = f"psr{a.psr_n}"
-        rp = measure_workload(wl, psr, a.psr_steps, 10, 5, a.tuning == "pinned")
+        rp = measure_workload(wl, psr, a.psr_steps, 10, 5, PINNED["psr"] if pin else None)
         print(f"[bench] {wl} done in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
         ps = summary(rp, rp["fp32_bulk"], wl)
         ps["steps"] = a.psr_steps
         ps["config"] = {"workload": f"{wl}: aligned rotator, dipole radius 20 cells, light cylinder 80 cells, "
                                     f"40-cell fixed absorbing layer, {a.psr_n}^3, CC=0.5",
                         "cells": psr.cells, "g": psr.g}
-        line["workloads"] = {wl: ps}
+        line.setdefault("workloads", {})[wl] = ps
+    if a.loop_n:
+        t0 = time.time()
+        loop = problems.loop(a.loop_n, steps=1000, driver=problems.LOOP_DRIVEN, host=False)
+        wl = f"loop{a.loop_n}"
+        rl = measure_workload(wl, loop, a.loop_steps, 10, 5, PINNED["loop"] if pin else None)
+        print(f"[bench] {wl} done in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+        ls = summary(rl, rl["fp32_bulk"], wl)
+        ls["steps"] = a.loop_steps
+        ls["tuning"] = {"fp32_bulk": rl["fp32_bulk"]["tuning"], "fp64": rl["fp64"]["tuning"]}
+        ls["config"] = {"workload": f"{wl}: twisted flux loop (user_loop field, footpoint driver on z <= 0, "
+                                    f"conducting top), {a.loop_n}^3, x/y periodic, z open, CC=0.45",
+                        "cells": loop.cells, "g": loop.g}
+        line.setdefault("workloads", {})[wl] = ls
     line["cpu_baseline"] = cpu
     print(json.dumps(line))
     return 0

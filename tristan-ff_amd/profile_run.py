@@ -2,9 +2,10 @@
 """Short, fixed workloads for hardware-counter and kernel-trace runs.
 
   step  -- a few fused timesteps of a benchmark workload at one precision:
-           alfven (B0 = z + seeded dB/B0 = 0.05, k <= 8, periodic) or psr
+           alfven (B0 = z + seeded dB/B0 = 0.05, k <= 8, periodic), psr
            (the aligned rotator: dipole of radius 20 cells, light cylinder
-           at 80 cells, 40-cell fixed absorbing layer on every side);
+           at 80 cells, 40-cell fixed absorbing layer on every side) or
+           loop (the driven twisted loop: x/y periodic, z open);
   copy  -- tff_init_rk_step on the same grid: two launches of a plain copy
            kernel (k_copy3), E -> En and B -> Bn over the allocated box,
            3 * nalloc words each way per launch -- a kernel of known
@@ -39,11 +40,13 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--tile", default="", help="tile:kchunk (default: autotuned)")
     ap.add_argument("--istep", type=int, default=2, help="output mode: node stride")
-    ap.add_argument("--workload", default="alfven", choices=("alfven", "psr"))
+    ap.add_argument("--workload", default="alfven", choices=("alfven", "psr", "loop"))
     ap.add_argument("--info", default="", help="write what ran as JSON to this file")
     a = ap.parse_args(argv)
     if a.workload == "alfven":
         p = problems.alfven(a.n, seed=1, kmax=8, host=False)
+    elif a.workload == "loop":
+        p = problems.loop(a.n, steps=1000, driver=problems.LOOP_DRIVEN, host=False)
     else:
         p = problems.psr(a.n, host=False)
     mb = pytff.Meshblock(p.n, boundary=p.boundary, cc=p.cc, precision=a.precision, absorb_buff=p.absorb_buff,

@@ -235,6 +235,13 @@ def scan_workload(count, steps, seed):
     out["energy_rel_err_max"] = float(np.max(np.abs(e32 - e64) / np.abs(e64)))
     out["delta_energy_rel_err_max"] = float(np.max(np.abs(d32 - d64) / np.maximum(scale, 1e-300)))
     out["batch_over_sequential_fp32_bulk"] = out["fp32_bulk"]["batch_cups"] / out["fp32_bulk"]["sequential_cups"]
+    out["batch_over_sequential_fp64"] = out["fp64"]["batch_cups"] / out["fp64"]["sequential_cups"]
+    # canonical HBM fraction: the algorithmic bytes of every instance's
+    # cell-updates (w (48 + 9 g) per cell-update) over the batch's wall time
+    for prec, word in (("fp64", 8), ("fp32_bulk", 4)):
+        bytes_step = sum(p.cells * word * (48 + 9 * p.g) for p in probs)
+        wall_step = out[prec]["batch_ms_per_step"] / 1e3
+        out[prec]["roofline_frac_canonical"] = bytes_step / wall_step / 8.0e12
     return out
 
 
