@@ -81,3 +81,22 @@ def test_host_false_builds_no_arrays():
               problems.psr(16, radius=3.0, r_lc=6.0, absorb_buff=4, host=False),
               problems.uniform(16, host=False)):
         assert p.fields is None and p.init is not None, p.name
+
+
+def test_park_miller_is_the_minimal_standard():
+    """The generator of the reference's randomNum (auxiliary.F90:246-257):
+    from seed 1, the Lehmer sequence x <- 16807 x mod (2^31 - 1) reaches
+    1043618065 at the 10000th draw (Park and Miller, CACM 31 (1988) 1192,
+    the published check value); each draw returns x / 2^31; the reference's
+    start is 123457 + rank."""
+    g = problems.ParkMiller()
+    assert g.seed == 123457.0
+    g.seed = 1.0
+    for _ in range(10000):
+        u = g.uniform()
+    assert g.seed == 1043618065.0
+    assert u == 1043618065.0 / 2147483648.0
+    # Box-Muller normals from it: mean ~ 0 and variance ~ 1 over many draws
+    h = problems.ParkMiller(5)
+    z = np.array([h.normal() for _ in range(20000)])
+    assert abs(z.mean()) < 0.03 and abs(z.var() - 1.0) < 0.03

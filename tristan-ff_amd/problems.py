@@ -123,14 +123,36 @@ def emwave_closed_form(p: Problem, amp, mode, nsteps, b0=1.0, axis="z"):
 
 
 # --- config: uniform B0 plus seeded Alfvenic perturbation -----------------
+class ParkMiller:
+    """The reference's generator (src/tools/auxiliary.F90:246-257,
+    randomNum): the Lehmer / Park-Miller minimal standard, seed <- 16807
+    seed mod (2^31 - 1), returning seed / 2^31, in FP64 as the reference
+    does (DMOD on doubles: every product is below 2^46, so exact).  The
+    seed starts at 123457 + rank (initializeRandomSeed, :296-301)."""
+
+    def __init__(self, rank: int = 0):
+        self.seed = 123457.0 + float(rank)
+
+    def uniform(self) -> float:
+        self.seed = math.fmod(16807.0 * self.seed, 2147483647.0)
+        return self.seed / 2147483648.0
+
+    def normal(self) -> float:
+        """One standard normal draw by the Box-Muller transform of two
+        uniforms (the cosine branch; u1 = 0 cannot occur)."""
+        u1, u2 = self.uniform(), self.uniform()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+
 def alfven_modes(n, seed=1, amp=0.05, kmax=8):
     """The seeded spectrum of alfven(): one row of 15 per wave vector k of
     the half space (kz > 0, or kz = 0 and ky > 0, or kz = ky = 0 and kx > 0)
     with 1 <= |k| <= kmax, holding kx ky kz and the complex amplitudes of
-    ex ey ez bx by bz.  Each k draws complex normal amplitudes for a vector
-    potential A and a stream function psi; B's are those of the forward-
-    difference curl of A (so div_h B = 0 to rounding) and E's those of the
-    forward-difference gradient of psi in the x-y plane.  With 2 kmax < n no
+    ex ey ez bx by bz.  Each k draws complex normal amplitudes (the
+    reference's Park-Miller generator started at rank `seed`, Box-Muller)
+    for a vector potential A and a stream function psi; B's are those of the
+    forward-difference curl of A (so div_h B = 0 to rounding) and E's those
+    of the forward-difference gradient of psi in the x-y plane.  With 2 kmax < n no
     two modes alias and none is its own conjugate, so the box mean of a
     mode's square is |c|^2 / 2 (Parseval) and the scales below make
     rms|dB| = rms|E| = amp exactly."""
@@ -143,8 +165,8 @@ def alfven_modes(n, seed=1, amp=0.05, kmax=8):
     k2 = kx * kx + ky * ky + kz * kz
     sel = half & (k2 >= 1) & (k2 <= kmax * kmax)
     kx, ky, kz = kx[sel], ky[sel], kz[sel]
-    rng = np.random.default_rng(seed)
-    z = rng.standard_normal((kx.size, 4, 2))
+    rng = ParkMiller(seed)
+    z = np.array([rng.normal() for _ in range(kx.size * 8)]).reshape(kx.size, 4, 2)
     z = z[..., 0] + 1j * z[..., 1]
     a, psi = z[:, :3], z[:, 3]
     # forward difference of exp(i theta) along each axis

@@ -18,9 +18,10 @@ the FP32-bulk path's error against the FP64 path on the same steps.
              after another on the single-meshblock path.
 
 Per workload: cell-updates/s of each path; for FP32-bulk, for FP32 (the
-reference's own precision) and for FP64 started from B perturbed by one
-part in 1e12 (how much the workload itself amplifies roundoff), against
-FP64: the energy history's relative
+reference's own precision), for the compensated state (FP32 pairs with
+two-sum stage accumulation, on the per-procedure kernels) and for FP64
+started from B perturbed by one part in 1e12 (how much the workload itself
+amplifies roundoff), against FP64: the energy history's relative
 error (on E_EM and on E_EM - E_EM(t=0)), the normwise max and L2 relative
 error of every component after the last step, and div_h B of each run at
 t = 0 and after the last step with its change over the run (over every
@@ -143,11 +144,12 @@ def closed_form_err(p, res, amp, mode, steps):
 
 
 def single_workload(name, p, steps, every, extra=None):
-    """FP64, FP32-bulk and FP32 (the reference's own precision) on the same
-    steps; the two FP32 paths are each compared with FP64."""
+    """FP64, FP32-bulk, FP32 (the reference's own precision) and the
+    compensated state (FP32 pairs, two-sum stage accumulation; per-procedure
+    kernels) on the same steps; each is compared with FP64."""
     log(f"{name}: {p.n[0]}^3, {steps} steps")
     runs = {}
-    for prec in ("fp64", "fp32_bulk", "fp32"):
+    for prec in ("fp64", "fp32_bulk", "fp32", "fp32_comp"):
         runs[prec] = run_single(p, prec, steps, every, True)
         log(f"  {prec} {runs[prec]['cups']:.4g} cell-updates/s")
     # FP64 again from B perturbed by one part in 1e12: how far roundoff-size
@@ -159,13 +161,14 @@ def single_workload(name, p, steps, every, extra=None):
     r64 = runs["fp64"]
     out = {"workload": name, "n": list(p.n), "steps": steps, "boundary": list(p.boundary), "cc": p.cc,
            "cells": p.cells, "g": p.g}
-    word = {"fp64": 8, "fp32_bulk": 4, "fp32": 4}
+    word = {"fp64": 8, "fp32_bulk": 4, "fp32": 4, "fp32_comp": 8}
     for prec, r in runs.items():
         out[prec] = {"cups": r["cups"], "ms_per_step": r["ms_per_step"], "tuning": r["tuning"],
                      "roofline_frac_canonical": r["cups"] * word[prec] * (48 + 9 * p.g) / 8.0e12}
     out["fp32_bulk_over_fp64"] = runs["fp32_bulk"]["cups"] / r64["cups"]
     out["fp32_bulk_vs_fp64"] = compare(p, r64, runs["fp32_bulk"])
     out["fp32_vs_fp64"] = compare(p, r64, runs["fp32"])
+    out["fp32_comp_vs_fp64"] = compare(p, r64, runs["fp32_comp"])
     out["fp64_perturbed_1e-12_vs_fp64"] = compare(p, r64, rpert)
     h = {prec: (r["hist"][:, 0] + r["hist"][:, 1]).tolist() for prec, r in runs.items()}
     h["fp64_perturbed_1e-12"] = (rpert["hist"][:, 0] + rpert["hist"][:, 1]).tolist()
