@@ -23,8 +23,9 @@ same way.
 
 The FP32-bulk energy is compared with the FP64 path's energy after the run.
 ``roofline`` is for the dominant kernel (the stage kernel of the FP32-bulk
-path): algorithmic bytes per launch over its average launch time, the time
-taken with HIP events on the library's own stream; ``traffic`` is the
+path): algorithmic bytes per launch over its average launch time, the timed
+region's HIP events (on the library's own stream) over its stage launches;
+``traffic`` is the
 calibrated FETCH_SIZE + WRITE_SIZE per launch stored under profiles/ by
 traffic.py, reported only when it was measured on the kernel sources being
 run (their hash is compared).
@@ -88,8 +89,17 @@ def run_path(precision: str, prob, steps: int, warmup: int, timing_steps: int, t
     e_end = mb.reduce()
     divb = problems.divb_report(prob, d0, problems.div_h_b(*(mb.download(f) for f in ("bx", "by", "bz"))), bmax)
     del d0
-    # dominant-kernel timing: the same steps again with every stage-kernel
-    # launch bracketed by events on the library's stream
+    # the stage kernel's launches in the timed region, read from HIP events
+    # on the library's stream around that region: its event time over the
+    # 3 * steps stage launches.  Whatever else the region launches (the
+    # near-null fix-up scans, ghost passes and hooks) counts as stage-kernel
+    # time, so this bounds the kernel's average duration from above; it
+    # agrees with rocprofv3's kernel trace of the same command within the
+    # fix-up scans' few microseconds.
+    region_launch_s = ms_events / 1e3 / (3 * steps)
+    # for reference: the same steps continued eagerly, each stage launch
+    # bracketed by events (eager launches run a few per cent slower than the
+    # graph-replayed ones)
     mb.kernel_timing(True)
     for s in range(timing_steps):
         mb.step(warmup + steps + s)
@@ -99,10 +109,10 @@ def run_path(precision: str, prob, steps: int, warmup: int, timing_steps: int, t
     word = 8 if precision == "fp64" else 4
     b_alg = algorithmic_bytes_per_cell_update(word, prob.g)
     cups = cells * steps / (ms / 1e3)
-    avg_launch_s = ktot / 1e3 / max(klaunch, 1)
+    eager_launch_s = ktot / 1e3 / max(klaunch, 1)
     # the three stage launches move 12 + 18 + 18 words per cell: 16 on average
     per_launch_bytes = cells * word * (48 + 9 * prob.g) / 3.0
-    achieved = per_launch_bytes / avg_launch_s
+    achieved = per_launch_bytes / region_launch_s
     tun = mb.get_tuning()
     mb.close()
     return {
@@ -113,9 +123,10 @@ def run_path(precision: str, prob, steps: int, warmup: int, timing_steps: int, t
         "tuning": tun,
         "b_alg_per_cell_update": b_alg,
         "roofline_frac_canonical": cups * b_alg / HBM_PEAK,
-        "stage_kernel": {"avg_ms": avg_launch_s * 1e3, "launches": klaunch,
+        "stage_kernel": {"avg_ms": region_launch_s * 1e3, "launches": 3 * steps,
                          "alg_bytes_per_launch": per_launch_bytes,
-                         "achieved_GBps": achieved / 1e9},
+                         "achieved_GBps": achieved / 1e9,
+                         "eager_event_avg_ms": eager_launch_s * 1e3, "eager_launches": klaunch},
         "energy_end": e_end[0] + e_end[1],
         "energy_start": e_start[0] + e_start[1],
         "divb": divb,
@@ -239,6 +250,8 @@ def roofline_of(head: dict, workload: str) -> dict:
         "kernel": "k_stage<PrecF32Bulk, %dx%d tile> (fused stage kernel, kchunk %d)" % (
             head["tuning"]["tx"], head["tuning"]["ty"], head["tuning"]["kchunk"]),
         "avg_launch_ms": head["stage_kernel"]["avg_ms"],
+        "avg_launch_source": "timed region's HIP events / stage launches (includes the fix-up scans)",
+        "eager_event_avg_ms": head["stage_kernel"]["eager_event_avg_ms"],
         "alg_bytes_per_launch": head["stage_kernel"]["alg_bytes_per_launch"],
     }
     tr, src = load_traffic(workload + "_fp32_bulk", pytff.kernel_source_hash())

@@ -233,7 +233,9 @@ int launch_rk3(tff_handle* h, const Vec& xe, const Vec& xb, const Vec& ue, const
   if (h->timing) {
     while (h->tpool.size() < h->tused + 2) {
       hipEvent_t e;
-      HIPCHK(h, hipEventCreate(&e));
+      // timing only: no system-scope fence (its cache writeback and
+      // invalidation would slow the kernel it brackets)
+      HIPCHK(h, hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
       h->tpool.push_back(e);
     }
     e0 = h->tpool[h->tused++];
@@ -552,7 +554,9 @@ int launch_stage(tff_handle* h, const StageArgs<typename PR::S, typename PR::P>&
   if (h->timing) {
     while (h->tpool.size() < h->tused + 2) {
       hipEvent_t e;
-      HIPCHK(h, hipEventCreate(&e));
+      // timing only: no system-scope fence (its cache writeback and
+      // invalidation would slow the kernel it brackets)
+      HIPCHK(h, hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
       h->tpool.push_back(e);
     }
     e0 = h->tpool[h->tused++];

@@ -224,6 +224,11 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
   const Geom& g = A.g;
   const int64_t cs = g.cs;
   const int tid = threadIdx.x;
+  // Halo cell of pass m of this thread's raw-plane loads: the first pass
+  // numbers the cells by thread, the later ones from the last thread down,
+  // so that a partial last pass falls on the last waves and not on the
+  // first ones, which also clean the halo ring.
+  auto halo_cell = [&](int m) { return m == 0 ? tid : m * NT + (NT - 1 - tid); };
   const int mode = WM == WRAP_RUNTIME ? A.wrap : WM;
   const bool wrap = mode == WRAP_PERIODIC;  // every axis wraps
   const bool mixed = mode == WRAP_MIXED;    // x and y wrap, z stored
@@ -236,7 +241,7 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
   uint32_t hxy[L2PT], hxs[L2PT];
 #pragma unroll
   for (int m = 0; m < L2PT; ++m) {
-    const int c = min(tid + m * NT, N2 - 1);
+    const int c = min(halo_cell(m), N2 - 1);
     const int ri = i0 - 2 + c % W2, rj = j0 - 2 + c / W2;
     const int si = min(ri, g.nx + NG - 1), sj = min(rj, g.ny + NG - 1);
     hxs[m] = (uint32_t)((g.org + si + (int64_t)sj * g.px) * (int64_t)sizeof(S));
@@ -267,7 +272,7 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
     const bool stored_plane = mixed && (p < 0 || p >= g.nz);
 #pragma unroll
     for (int m = 0; m < L2PT; ++m) {
-      if (tid + m * NT < N2) {
+      if (halo_cell(m) < N2) {
         const uint32_t o = stored_plane ? hxs[m] : hxy[m];
         reg[m][0] = ld_at(e0, o);
         reg[m][1] = ld_at(e1, o);
@@ -281,7 +286,7 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
   auto store_plane = [&](int ps) {
 #pragma unroll
     for (int m = 0; m < L2PT; ++m) {
-      const int c = tid + m * NT;
+      const int c = halo_cell(m);
       if (c < N2) {
         rex[slot<RG::EX>(ps) * N2 + c] = reg[m][0];
         rey[slot<RG::EY>(ps) * N2 + c] = reg[m][1];

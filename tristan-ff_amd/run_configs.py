@@ -187,14 +187,21 @@ def scan_workload(count, steps, seed):
     out = {"workload": f"scan{count}", "instances": count, "cells": cells, "steps": steps,
            "edges": [p.n[0] for p in probs], "kinds": [p.name.rstrip("0123456789") for p in probs]}
     hists = {}
-    for prec in ("fp64", "fp32_bulk"):
-        b = pytff.Batch(problems.batch_specs(probs), precision=prec)
-        for q, p in enumerate(probs):
+    # FP64 again from B perturbed by one part in 1e12 (batched): how far
+    # roundoff-size differences grow on each instance by themselves
+    pert = []
+    for p in probs:
+        pp = dataclasses.replace(p, fields=dict(p.fields))
+        pp.fields["bz"] = p.fields["bz"] * (1.0 + 1e-12)
+        pert.append(pp)
+    for prec in ("fp64", "fp32_bulk", "fp64_perturbed"):
+        b = pytff.Batch(problems.batch_specs(probs), precision=prec.replace("_perturbed", ""))
+        for q, p in enumerate(pert if prec == "fp64_perturbed" else probs):
             problems.load(b[q], p)
             b[q].preloop()
         h0 = [b[q].reduce() for q in range(count)]
         d0, bmax = [], []
-        for q in range(count):
+        for q in range(count if prec != "fp64_perturbed" else 0):
             bq = [b[q].download(f) for f in ("bx", "by", "bz")]
             bmax.append(max(float(np.abs(_interior(a)).max()) for a in bq))
             d0.append(problems.div_h_b(*bq))
@@ -205,6 +212,9 @@ def scan_workload(count, steps, seed):
             b[q].sync()
         wall = time.perf_counter() - t
         hists[prec] = (np.array(h0), np.array([b[q].reduce() for q in range(count)]))
+        if prec == "fp64_perturbed":
+            b.close()
+            continue
         reps = [problems.divb_report(p, d0[q], problems.div_h_b(*(b[q].download(f) for f in ("bx", "by", "bz"))),
                                      bmax[q]) for q, p in enumerate(probs)]
         del d0
@@ -236,6 +246,10 @@ def scan_workload(count, steps, seed):
     scale = np.maximum(np.abs(d64), np.array([perturbation_energy(p) for p in probs]))
     scale = np.maximum(scale, 1e-12 * np.abs(e64))
     out["energy_rel_err_max"] = float(np.max(np.abs(e32 - e64) / np.abs(e64)))
+    ep = hists["fp64_perturbed"][1][:, 0] + hists["fp64_perturbed"][1][:, 1]
+    out["energy_rel_err_per_instance"] = [float(x) for x in np.abs(e32 - e64) / np.abs(e64)]
+    out["perturbed_fp64_energy_rel_err_per_instance"] = [float(x) for x in np.abs(ep - e64) / np.abs(e64)]
+    out["perturbed_fp64_energy_rel_err_max"] = float(np.max(np.abs(ep - e64) / np.abs(e64)))
     out["delta_energy_rel_err_max"] = float(np.max(np.abs(d32 - d64) / np.maximum(scale, 1e-300)))
     out["batch_over_sequential_fp32_bulk"] = out["fp32_bulk"]["batch_cups"] / out["fp32_bulk"]["sequential_cups"]
     out["batch_over_sequential_fp64"] = out["fp64"]["batch_cups"] / out["fp64"]["sequential_cups"]
