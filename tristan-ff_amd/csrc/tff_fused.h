@@ -16,12 +16,12 @@
 // plane is stored: ex, ey raw and cleaned 2, ez raw and cleaned 3, bz 3,
 // bx and by 4 (the curl of plane k reads plane k-1).
 //
-// Per iteration (output plane k), three barriers: store raw plane k+2
-// (loaded into registers during the previous iteration) and issue the loads
-// of plane k+3 and of U^n at plane k, so that their latency overlaps this
-// iteration's work | clean plane k+1 on the tile and its halo | rk3Push /
-// rk3Update on plane k, with rho evaluated at the four nodes each cell's
-// currents read | ring slots free.  The per-cell arithmetic is
+// Per iteration (output plane k), two barriers (three in the fix-up kernel,
+// see Ring): store raw plane k+2 (loaded into registers during the previous
+// iteration) and issue the loads of plane k+3 and of U^n at plane k, so
+// that their latency overlaps this iteration's work | clean plane k+1 on the
+// tile and its halo | rk3Push / rk3Update on plane k, with rho evaluated at
+// the four nodes each cell's currents read.  The per-cell arithmetic is
 // tff_cell.h's, so results equal the unfused kernels bit for bit.
 //
 // FP32-bulk: an output plane whose stencil reads a near-null cell is not
@@ -142,10 +142,19 @@ __device__ __forceinline__ S ld_at(const S* base, uint32_t off) {
   asm volatile("" : "+v"(off));
   return *(const S*)((const char*)base + off);
 }
+// Streaming forms, for data that no other workgroup of the launch reads:
+// U^n and the B0 curl (read once, at the thread's own cell) and the stage
+// outputs.  The non-temporal hint keeps them from displacing the raw-plane
+// lines in L2, whose halos neighbouring tiles read again a few planes later.
+template <class S>
+__device__ __forceinline__ S ld_once(const S* base, uint32_t off) {
+  asm volatile("" : "+v"(off));
+  return __builtin_nontemporal_load((const S*)((const char*)base + off));
+}
 template <class S>
 __device__ __forceinline__ void st_at(S* base, uint32_t off, S v) {
   asm volatile("" : "+v"(off));
-  *(S*)((char*)base + off) = v;
+  __builtin_nontemporal_store(v, (S*)((char*)base + off));
 }
 
 // LDS accessor of one field around one cell: p[d+1] points at neighbour
@@ -462,18 +471,18 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
       if (!A.stage1) {
         const S* ue = A.ue + pk;
         const S* ub = A.ub + pk;
-        un[0] = ld_at(ue, oxy);
-        un[1] = ld_at(ue + cs, oxy);
-        un[2] = ld_at(ue + 2 * cs, oxy);
-        un[3] = ld_at(ub, oxy);
-        un[4] = ld_at(ub + cs, oxy);
-        un[5] = ld_at(ub + 2 * cs, oxy);
+        un[0] = ld_once(ue, oxy);
+        un[1] = ld_once(ue + cs, oxy);
+        un[2] = ld_once(ue + 2 * cs, oxy);
+        un[3] = ld_once(ub, oxy);
+        un[4] = ld_once(ub + cs, oxy);
+        un[5] = ld_once(ub + 2 * cs, oxy);
       }
       if (A.has_t0) {
         const S* t0 = A.t0 + pk;
-        tt[0] = (T)ld_at(t0, oxy);
-        tt[1] = (T)ld_at(t0 + cs, oxy);
-        tt[2] = (T)ld_at(t0 + 2 * cs, oxy);
+        tt[0] = (T)ld_once(t0, oxy);
+        tt[1] = (T)ld_once(t0 + cs, oxy);
+        tt[2] = (T)ld_once(t0 + 2 * cs, oxy);
       }
     };
     if (EARLY && active) load_un();
