@@ -794,7 +794,11 @@ template <class PR>
 int autotune_t(tff_handle* h) {
   using S = typename PR::S;
   if (h->tile >= 0) return TFF_OK;
-  const int kcs[] = {8, 16, 32, 64};
+  // z-chunks of 4 planes spread a small box (up to 128^3) over more
+  // workgroups: a tile's planes are a chain of dependent iterations, so on a
+  // box that does not fill the CUs the chain length sets the launch time
+  const int kcs[] = {4, 8, 16, 32, 64};
+  const bool small = (int64_t)h->g.nx * h->g.ny * h->g.nz <= (int64_t)128 * 128 * 128;
   float best = 1e30f;
   int bt = -1, bk = 0;
   hipEvent_t e0, e1;
@@ -805,7 +809,7 @@ int autotune_t(tff_handle* h) {
   for (int t = 0; t < NTILES; ++t) {
     if (tile_lds<PR>(t) > (size_t)h->max_lds) continue;
     for (int kc : kcs) {
-      if (kc > 2 * h->g.nz && kc != 8) continue;
+      if ((kc > 2 * h->g.nz && kc != 8) || (kc == 4 && !small)) continue;
       h->kchunk = kc;
       auto a = stage_common<PR>(h);
       set_coef<PR>(h, a, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0);
