@@ -65,16 +65,25 @@ struct Tile {
 // near-null cells from the raw B) the push reads raw B at k-1..k+1 and a
 // third barrier protects the rings.
 //
-// STATIC (FP32 storage with TWO): every ring holds 2 or 4 planes, so with
-// the z loop unrolled by four (z-chunks start at multiples of 4) every ring
-// slot is a compile-time constant: LDS addresses are one base register
-// plus immediate offsets, and the own-cell registers alternate between two
-// sets instead of being copied.  FP64 keeps rings of 3 (the LDS of a
-// second workgroup per CU) and runtime slots.
-template <class PR, bool FIX>
+// STATIC (with TWO): every ring holds 2 or 4 planes, so with the z loop
+// unrolled by four (z-chunks start at multiples of 4) every ring slot is a
+// compile-time constant: LDS addresses are one base register plus
+// immediate offsets, and the own-cell registers alternate between two sets
+// instead of being copied.  FP32 storage always takes it; FP64 takes it
+// where the deeper rings leave as many workgroups per CU as rings of 3
+// (64-wide and 32x16 tiles run one FP64 workgroup per CU either way), and
+// otherwise keeps rings of 3 with runtime slots.
+template <class PR, bool FIX, class TL>
 struct Ring {
   static constexpr bool TWO = !(PR::null_fp64 && !FIX);
-  static constexpr bool STATIC = TWO && sizeof(typename PR::S) == 4;
+  // LDS bytes of TWO rings whose ez, bx, by and cleaned-ez rings hold d
+  // planes (the others 2), with 8 flag words
+  static constexpr size_t two_bytes(int d) {
+    return sizeof(typename PR::S) * (size_t)((6 + 3 * d) * TL::N2 + (4 + d) * TL::N1) + 8 * sizeof(unsigned);
+  }
+  static constexpr size_t CU_BYTES = 160 * 1024;
+  static constexpr bool STATIC =
+      TWO && (sizeof(typename PR::S) == 4 || CU_BYTES / two_bytes(4) == CU_BYTES / two_bytes(3));
   static constexpr int EX = 2, EY = 2, EZ = STATIC ? 4 : 3;
   static constexpr int BX = STATIC ? 4 : (TWO ? 3 : 4), BY = BX, BZ = TWO ? 2 : 3;
   static constexpr int CX = 2, CY = 2, CZ = STATIC ? 4 : 3;
@@ -82,7 +91,6 @@ struct Ring {
   static constexpr int CLEAN = CX + CY + CZ;
   // near-null flags of the planes (FP32-bulk), one word per plane slot
   static constexpr int FLAGS = TWO ? 8 : 4;
-  template <class TL>
   static constexpr size_t bytes() {
     return sizeof(typename PR::S) * (size_t)(RAW * TL::N2 + CLEAN * TL::N1) + FLAGS * sizeof(unsigned);
   }
@@ -91,7 +99,7 @@ struct Ring {
 // LDS of a stage_run<PR, TL, FIX> workgroup
 template <class PR, class TL, bool FIX>
 constexpr size_t stage_lds() {
-  return Ring<PR, FIX>::template bytes<TL>();
+  return Ring<PR, FIX, TL>::bytes();
 }
 // the larger of the stage kernel's and its fix-up kernel's LDS
 template <class PR, class TL>
@@ -214,7 +222,7 @@ __device__ __forceinline__ void stage_run(const StageArgs<typename PR::S, typena
   using T = typename PR::T;
   using Q = typename PR::Q;
   using P = typename PR::P;
-  using RG = Ring<PR, FIX>;
+  using RG = Ring<PR, FIX, TL>;
   constexpr int TX = TL::TX, TY = TL::TY, W2 = TL::W2, N2 = TL::N2, W1 = TL::W1, N1 = TL::N1;
   constexpr int NT = TL::NT, L2PT = TL::L2PT;
   constexpr bool TWO = RG::TWO;
