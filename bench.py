@@ -216,7 +216,7 @@ def load_traffic(workload: str, source_hash: str):
 # 1 32x8, 2 64x4, 3 64x8, 4 32x16, 5 64x16.
 PINNED = {
     "alfven": {"fp64": (3, 64), "fp32_bulk": (3, 64)},
-    "psr": {"fp64": (1, 32), "fp32_bulk": (4, 32)},
+    "psr": {"fp64": (3, 32), "fp32_bulk": (4, 32)},
     "loop": {"fp64": (3, 64), "fp32_bulk": (4, 16)},
 }
 
