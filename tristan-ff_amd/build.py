@@ -46,6 +46,25 @@ def build_lib(verbose: bool = False) -> str:
     return LIB
 
 
+PROBE = os.path.join(OUT, "stream_probe")
+
+
+def build_probe(verbose: bool = False) -> str:
+    """stream_probe: the streaming bandwidth of the stage kernel's mix of HBM
+    streams with no arithmetic (the ceiling its roofline fraction is read
+    against)."""
+    os.makedirs(OUT, exist_ok=True)
+    src = os.path.join(PKG, "csrc", "stream_probe.hip")
+    tmp = PROBE + ".tmp"
+    cmd = [HIPCC, "-O3", f"--offload-arch={ARCH}", "-std=c++17",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", src, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, PROBE)
+    return PROBE
+
+
 HDF5_ROOT = os.environ.get("HDF5_ROOT", "/opt/conda")
 IO_LIB = os.path.join(OUT, "libtff_io.so")
 
@@ -92,6 +111,7 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     print(build_lib(a.verbose))
     print(build_io(a.verbose))
+    print(build_probe(a.verbose))
     if a.fortran and shutil.which("amdflang"):
         print(build_fortran(a.verbose))
     return 0
