@@ -107,12 +107,27 @@ __device__ __forceinline__ double clean_point_fp64(const AX& EX, const AY& EY, c
   return e0 - epar * b0 * rcp_nr(bmag);
 }
 
+// The cleaned E component at E point P (field_solver.F90:288-328) given
+// the B interpolants there (bx_, by_, bz_) and their |B|^2: E along its own
+// axis at the point, the other two as 4-point averages in T2; epar = E.B in
+// Q2, summed in the order x, y, z; E <- E - epar B / |B|^2 (no TINY).  With
+// `guard`, |B|^2 == 0 leaves E unchanged.
+template <int P, class T2, class Q2, bool FAST, class AX, class AY, class AZ>
+__device__ __forceinline__ Q2 clean_point(const AX& EX, const AY& EY, const AZ& EZ, Q2 bx_, Q2 by_, Q2 bz_,
+                                          Q2 bmag, int guard) {
+  T2 intex, intey, intez;
+  e_point<P, T2>(EX, EY, EZ, intex, intey, intez);
+  const Q2 epar = FAST ? mad_<FAST>((Q2)intez, bz_, mad_<FAST>((Q2)intey, by_, (Q2)intex * bx_))
+                       : ((Q2)intex * bx_ + (Q2)intey * by_ + (Q2)intez * bz_);
+  const Q2 e0 = P == 0 ? (Q2)EX(0, 0, 0) : (P == 1 ? (Q2)EY(0, 0, 0) : (Q2)EZ(0, 0, 0));
+  const Q2 b0 = P == 0 ? bx_ : (P == 1 ? by_ : bz_);
+  return (guard && bmag == (Q2)0) ? e0 : (FAST ? e0 - div_<FAST>(epar * b0, bmag) : e0 - epar * b0 / bmag);
+}
+
 // cleanEpar at one cell (field_solver.F90:288-328) with sums in T2 and the
-// projection in Q2, given the B interpolants: returns the three cleaned E
-// components.  E along its own axis at the point, the other two as 4-point
-// averages; epar = E.B; E <- E - epar B / |B|^2 (no TINY).  With `guard`,
-// |B|^2 == 0 leaves E unchanged.  `near` reports a point whose |B|^2 is
-// below NEAR_NULL times `bref` (when bref > 0).
+// projection in Q2, given the B interpolants: the three cleaned E
+// components, clean_point at each E point.  `near` reports a point whose
+// |B|^2 is below NEAR_NULL times `bref` (when bref > 0).
 template <class T2, class Q2, bool FAST, class AX, class AY, class AZ>
 __device__ __forceinline__ void clean_impl(const AX& EX, const AY& EY, const AZ& EZ, const BBar<Q2>& bb_in,
                                            int guard, Q2 bref, Q2& ox, Q2& oy, Q2& oz, bool& near) {
@@ -137,36 +152,11 @@ __device__ __forceinline__ void clean_impl(const AX& EX, const AY& EY, const AZ&
       }
     }
   }
-  {
-    T2 intex, intey, intez;
-    e_point<0, T2>(EX, EY, EZ, intex, intey, intez);
-    const Q2 bx_ = bb.x[0], by_ = bb.y[0], bz_ = bb.z[0], bmag = bb.m[0];
-    Q2 epar = FAST ? mad_<FAST>((Q2)intez, bz_, mad_<FAST>((Q2)intey, by_, (Q2)intex * bx_))
-                   : ((Q2)intex * bx_ + (Q2)intey * by_ + (Q2)intez * bz_);
-    near |= bb_in.m[0] < (Q2)NEAR_NULL * bref;
-    const Q2 e0 = (Q2)EX(0, 0, 0);
-    ox = (guard && bmag == (Q2)0) ? e0 : (FAST ? e0 - div_<FAST>(epar * bx_, bmag) : e0 - epar * bx_ / bmag);
-  }
-  {
-    T2 intex, intey, intez;
-    e_point<1, T2>(EX, EY, EZ, intex, intey, intez);
-    const Q2 bx_ = bb.x[1], by_ = bb.y[1], bz_ = bb.z[1], bmag = bb.m[1];
-    Q2 epar = FAST ? mad_<FAST>((Q2)intez, bz_, mad_<FAST>((Q2)intey, by_, (Q2)intex * bx_))
-                   : ((Q2)intex * bx_ + (Q2)intey * by_ + (Q2)intez * bz_);
-    near |= bb_in.m[1] < (Q2)NEAR_NULL * bref;
-    const Q2 e0 = (Q2)EY(0, 0, 0);
-    oy = (guard && bmag == (Q2)0) ? e0 : (FAST ? e0 - div_<FAST>(epar * by_, bmag) : e0 - epar * by_ / bmag);
-  }
-  {
-    T2 intex, intey, intez;
-    e_point<2, T2>(EX, EY, EZ, intex, intey, intez);
-    const Q2 bx_ = bb.x[2], by_ = bb.y[2], bz_ = bb.z[2], bmag = bb.m[2];
-    Q2 epar = FAST ? mad_<FAST>((Q2)intez, bz_, mad_<FAST>((Q2)intey, by_, (Q2)intex * bx_))
-                   : ((Q2)intex * bx_ + (Q2)intey * by_ + (Q2)intez * bz_);
-    near |= bb_in.m[2] < (Q2)NEAR_NULL * bref;
-    const Q2 e0 = (Q2)EZ(0, 0, 0);
-    oz = (guard && bmag == (Q2)0) ? e0 : (FAST ? e0 - div_<FAST>(epar * bz_, bmag) : e0 - epar * bz_ / bmag);
-  }
+#pragma unroll
+  for (int p = 0; p < 3; ++p) near |= bb_in.m[p] < (Q2)NEAR_NULL * bref;
+  ox = clean_point<0, T2, Q2, FAST>(EX, EY, EZ, bb.x[0], bb.y[0], bb.z[0], bb.m[0], guard);
+  oy = clean_point<1, T2, Q2, FAST>(EX, EY, EZ, bb.x[1], bb.y[1], bb.z[1], bb.m[1], guard);
+  oz = clean_point<2, T2, Q2, FAST>(EX, EY, EZ, bb.x[2], bb.y[2], bb.z[2], bb.m[2], guard);
 }
 
 // |B|^2 of the raw cell, the scale of the near-null test
