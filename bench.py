@@ -211,13 +211,14 @@ def load_traffic(workload: str, source_hash: str):
 
 
 # Stage-kernel shapes the benchmarks pin, (tile index, z-chunk) per path:
-# the autotuner's usual pick on each workload, so that the line and a
-# profile of the same command time the same kernel.  Tile indices: 0 32x4,
-# 1 32x8, 2 64x4, 3 64x8, 4 32x16, 5 64x16.
+# the fastest shape of a sweep of tiles 0-5 with z-chunks 16, 32 and 64 on
+# each workload (two interleaved rounds, 40 graph-replayed steps each), so
+# that the line and a profile of the same command time the same kernel.
+# Tile indices: 0 32x4, 1 32x8, 2 64x4, 3 64x8, 4 32x16, 5 64x16.
 PINNED = {
     "alfven": {"fp64": (3, 64), "fp32_bulk": (3, 64)},
-    "psr": {"fp64": (3, 32), "fp32_bulk": (4, 32)},
-    "loop": {"fp64": (3, 64), "fp32_bulk": (4, 16)},
+    "psr": {"fp64": (1, 32), "fp32_bulk": (4, 32)},
+    "loop": {"fp64": (1, 64), "fp32_bulk": (4, 16)},
 }
 
 
