@@ -804,46 +804,89 @@ int autotune_t(tff_handle* h) {
   hipEvent_t e0, e1;
   HIPCHK(h, hipEventCreate(&e0));
   HIPCHK(h, hipEventCreate(&e1));
-  const bool saved_timing = h->timing;
+  // no per-launch or per-phase events while the search launches (nor inside
+  // its captured graphs); the search's time is not the steps' time
+  const bool saved_timing = h->timing, saved_phases = h->phases;
   h->timing = false;
+  h->phases = false;
+  // every (tile, z-chunk) the box and the device admit
+  std::vector<std::pair<int, int>> cand;
   for (int t = 0; t < NTILES; ++t) {
     if (tile_lds<PR>(t) > (size_t)h->max_lds) continue;
     for (int kc : kcs) {
       if ((kc > 2 * h->g.nz && kc != 8) || (kc == 4 && !small)) continue;
-      h->kchunk = kc;
-      auto a = stage_common<PR>(h);
-      set_coef<PR>(h, a, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0);
-      a.ye = base_c<S>(h->es);
-      a.yb = base_c<S>(h->bs);
-      a.ge = base_c<S>(h->es);
-      a.clean_in = 1;
-      a.ue = base_c<S>(h->es);
-      a.ub = base_c<S>(h->bs);
-      a.oe = base_m<S>(h->w2e);
-      a.ob = base_m<S>(h->w2b);
-      int rc = launch_stage<PR>(h, a, t);  // warm-up (and instruction cache)
-      if (rc) return rc;
-      // the faster of two rounds of three launches (shapes within a few
-      // per cent of each other would otherwise swap places run to run)
-      float ms = 1e30f;
-      for (int round = 0; round < 2; ++round) {
-        HIPCHK(h, hipEventRecord(e0, h->stream));
-        for (int r = 0; r < 3; ++r)
-          if ((rc = launch_stage<PR>(h, a, t))) return rc;
-        HIPCHK(h, hipEventRecord(e1, h->stream));
-        HIPCHK(h, hipEventSynchronize(e1));
-        float m = 0.f;
-        HIPCHK(h, hipEventElapsedTime(&m, e0, e1));
-        ms = std::min(ms, m);
-      }
-      if (ms < best) {
-        best = ms;
-        bt = t;
-        bk = kc;
-      }
+      cand.emplace_back(t, kc);
     }
   }
+  // Two passes over the candidates, each timing one replay of a graph of
+  // three launches of every shape (captured after one eager warm-up launch
+  // in the first pass); a shape keeps its faster pass.  Replayed graphs
+  // rank the shapes as the graph-replayed steps run them (eager launches
+  // favoured shorter z-chunks), the faster of two passes keeps shapes
+  // within a few per cent of each other from swapping places run to run,
+  // and timing the passes one after the other keeps a drift of the GPU's
+  // clocks during the search from favouring the shapes timed last.
+  std::vector<float> ms(cand.size(), 1e30f);
+  std::vector<hipGraphExec_t> execs(cand.size(), nullptr);
+  auto drop_execs = [&]() {
+    for (auto& x : execs)
+      if (x) hipGraphExecDestroy(x);
+  };
+  int rc = 0;
+  for (int pass = 0; pass < 2 && rc == 0; ++pass) {
+    for (size_t c = 0; c < cand.size() && rc == 0; ++c) {
+      const int t = cand[c].first;
+      h->kchunk = cand[c].second;
+      if (pass == 0) {
+        auto a = stage_common<PR>(h);
+        set_coef<PR>(h, a, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0);
+        a.ye = base_c<S>(h->es);
+        a.yb = base_c<S>(h->bs);
+        a.ge = base_c<S>(h->es);
+        a.clean_in = 1;
+        a.ue = base_c<S>(h->es);
+        a.ub = base_c<S>(h->bs);
+        a.oe = base_m<S>(h->w2e);
+        a.ob = base_m<S>(h->w2b);
+        if ((rc = launch_stage<PR>(h, a, t))) break;  // warm-up (instruction cache, kernel attributes)
+        hipGraph_t graph = nullptr;
+        if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+          rc = fail(h, TFF_EHIP, "autotune: hipStreamBeginCapture failed");
+          break;
+        }
+        for (int r = 0; r < 3 && rc == 0; ++r) rc = launch_stage<PR>(h, a, t);
+        const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+        if (rc == 0 && e != hipSuccess) rc = fail(h, TFF_EHIP, "autotune: hipStreamEndCapture failed");
+        if (rc == 0 && hipGraphInstantiate(&execs[c], graph, nullptr, nullptr, 0) != hipSuccess)
+          rc = fail(h, TFF_EHIP, "autotune: hipGraphInstantiate failed");
+        if (graph) hipGraphDestroy(graph);
+        if (rc) break;
+      }
+      float m = 0.f;
+      if (hipEventRecord(e0, h->stream) != hipSuccess || hipGraphLaunch(execs[c], h->stream) != hipSuccess ||
+          hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&m, e0, e1) != hipSuccess) {
+        rc = fail(h, TFF_EHIP, "autotune: timed replay failed");
+        break;
+      }
+      ms[c] = std::min(ms[c], m);
+    }
+  }
+  drop_execs();
   h->timing = saved_timing;
+  h->phases = saved_phases;
+  if (rc) {
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+  }
+  for (size_t c = 0; c < cand.size(); ++c) {
+    if (ms[c] < best) {
+      best = ms[c];
+      bt = cand[c].first;
+      bk = cand[c].second;
+    }
+  }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (bt < 0) return fail(h, TFF_EINVAL, "no tile shape fits the device's LDS");
